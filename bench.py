@@ -1,0 +1,144 @@
+"""Benchmark: HSMU SpGEMM C = A*A on one MI355X, banded FEM-like workload.
+
+Workload: A 62,451 x 62,451, per-row nnz uniform in [40, 88], distinct columns uniform
+within +-400 of the diagonal, values N(0,1), seed 1 (the cant-like configuration the
+metric is quoted on).  A step is one full SpGEMM through the C ABI (B -> tiles, row
+analysis, symbolic, allocation of C from the context pool, numeric), with A already
+resident in HBM; C is released back to the pool after each step.
+
+Prints one JSON line.  value = 2 * products / step time (GFLOP/s, whole job).
+roofline: the dominant kernel (the numeric kernel of the bin with the most time),
+achieved = its algorithmic bytes (gather model: A row extent + A col/val + B row extent
+per a_ij + one B col/val per product + C row pointer + C col/val, over its rows) / its
+mean duration from HIP events on the library stream.  traffic: HBM bytes per launch from
+the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present.
+cpu_baseline: the CPU oracle (row-wise Gustavson, C + OpenMP) on the same workload,
+repeated for about 10 s, median.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "hsmu-spgemm_amd"))
+
+PEAK_HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_numeric_banded.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
+              f"(the path has no multi-GPU mode)", file=sys.stderr)
+        sys.exit(2)
+
+    import hsmu
+
+    t0 = time.time()
+    A = hsmu.gen_banded(62451, 40, 88, 400, seed=1)
+    log(f"generated banded A: {A.M} rows, nnz {A.nnz} in {time.time() - t0:.1f}s")
+    ctx = hsmu.Context(0)
+    dA = ctx.matrix(A)
+    opts = hsmu.default_opts()
+
+    for _ in range(args.warmup):
+        c, st = ctx.spgemm_device(dA, dA, opts)
+        ctx.free_result(c)
+    hsmu.lib.hsmu_sync(ctx.handle)
+
+    stats = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        c, st = ctx.spgemm_device(dA, dA, opts)
+        ctx.free_result(c)
+        stats.append(st)
+    hsmu.lib.hsmu_sync(ctx.handle)
+    elapsed = time.perf_counter() - t_start
+    ms_step = elapsed * 1e3 / args.steps
+
+    st0 = stats[0]
+    products = st0["products"]
+    nnz_c = st0["nnz_c"]
+    gflops = 2.0 * products / (ms_step * 1e6)
+    # dominant kernel: numeric bin with the largest mean time
+    nb = len(st0["ms_numeric_bin"])
+    mean_bin = [statistics.mean(s["ms_numeric_bin"][b] for s in stats) for b in range(nb)]
+    dom = max(range(nb), key=lambda b: mean_bin[b])
+    dom_ms = mean_bin[dom]
+    dom_bytes = st0["num_bin_bytes"][dom]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    e2e_gbs = st0["alg_bytes"] / (ms_step * 1e-3) / 1e9
+    stage = {k: round(statistics.median(s[k] for s in stats), 4)
+             for k in ("ms_conversion", "ms_form_tileCptr", "ms_tileOR", "ms_malloc", "ms_form_cval",
+                       "ms_time2", "ms_total")}
+    log(f"step {ms_step:.3f} ms, {gflops:.1f} GFLOP/s, end-to-end {e2e_gbs:.0f} GB/s algorithmic; "
+        f"stages {stage}; numeric bins {st0['num_bin_rows']}; dominant bin {dom}: {dom_ms:.3f} ms, "
+        f"{achieved:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        runs = []
+        t_cpu = time.perf_counter()
+        while True:
+            t1 = time.perf_counter()
+            oracle.spgemm(A, A, want_abs=False)
+            runs.append(time.perf_counter() - t1)
+            if len(runs) >= 3 and time.perf_counter() - t_cpu > args.cpu_seconds:
+                break
+        med = statistics.median(runs)
+        cpu = {"value": round(2.0 * products / (med * 1e9), 3), "unit": "GFLOP/s",
+               "cores": oracle.max_threads(), "kind": "port",
+               "sample": f"full workload (C=A*A, all {A.M} rows) repeated {len(runs)} times, "
+                         f"median {med * 1e3:.1f} ms, row-wise Gustavson oracle (C, OpenMP)"}
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "SpGEMM GFLOP/s (2*products/time)",
+        "value": round(gflops, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "banded FEM-like C=A*A: 62451 rows, nnz/row U[40,88], columns within +-400, "
+                               "N(0,1) values, seed 1",
+                   "rows": A.M, "nnz_a": A.nnz, "products": products, "nnz_c": nnz_c,
+                   "end_to_end_alg_GBps": round(e2e_gbs, 1), "stage_ms": stage},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": f"k_numeric_lds bin {dom}" if dom < 8 else "k_numeric_global",
+                     "kernel_ms": round(dom_ms, 4), "alg_bytes_per_launch": dom_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    dA.free()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

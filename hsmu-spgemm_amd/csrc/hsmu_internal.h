@@ -1,0 +1,132 @@
+// hsmu_internal.h -- declarations shared by the kernels (kernels.hip) and the host
+// orchestrator (api.cpp).  Not part of the public ABI.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+namespace hsmu {
+
+// Row bins.  Bins 0..7 keep a row in LDS with G = 8 << b lanes per row (rows per
+// workgroup = max(1, 256 / G)).  BIN_GLOBAL keeps the row's tile directory in global
+// memory and accumulates straight into C (rows whose LDS need exceeds the limit).
+constexpr int NLDS = 8;
+constexpr int BIN_GLOBAL = 8;
+constexpr int NBINS = 10;  // slot 9 reserved
+constexpr int BLOCK = 256; // workgroup size of the multi-row bins
+
+// error bits raised on the device, reported at the next host synchronisation
+constexpr int ERR_ROWPTR = 1;   // row pointer decreasing or out of [0, nnz]
+constexpr int ERR_ACOL = 2;     // column of A outside [0, rows of B)
+constexpr int ERR_BCOL = 4;     // column of B outside [0, N)
+constexpr int ERR_UNSORTED = 8; // row of B with decreasing columns
+constexpr int ERR_INTERNAL = 16;// symbolic and numeric counts disagree (a bug)
+
+struct BinParams {
+    long long lds_block_budget; // LDS bytes per workgroup for bins with several rows per workgroup
+    long long lds_row_max;      // LDS bytes one row may use
+    long long per_lane;         // work items per lane
+    int force_global;
+    int pad;
+};
+
+struct BinStats {
+    int count[NBINS];
+    int max_need[NBINS]; // LDS bytes of the largest row in the bin
+    int max_span[NBINS]; // tiles spanned by the widest row in the bin
+    unsigned long long work[NBINS]; // products (numeric) or tile-products (symbolic)
+    unsigned long long nnz_c[NBINS];// nnz of C in the bin's rows (numeric)
+    unsigned long long nnz_a[NBINS];// nnz of A in the bin's rows
+};
+
+// Device-side counters of one call, copied to pinned host memory at the two
+// synchronisation points.
+struct DevStats {
+    unsigned long long products;
+    unsigned long long tile_products;
+    unsigned long long rows_nonempty;
+    long long nnz_c;
+    int err;
+    int ntiles_b;
+    int max_span;
+    int pad;
+    BinStats sym;
+    BinStats num;
+};
+
+// LDS bytes a row needs in each phase; S = tiles spanned, n = nnz of the row of C.
+// Symbolic: one 64-bit mask per tile.  Numeric: mask + 32-bit offset per tile and one
+// FP64 (or two-float) accumulator per nonzero.
+#if defined(__HIPCC__)
+#define HSMU_HD __host__ __device__
+#else
+#define HSMU_HD
+#endif
+// (rounded up to 16 bytes: per-row slices of a workgroup stay 16-byte aligned)
+HSMU_HD inline long long sym_need(long long S) { return (8 * S + 15) & ~15LL; }
+HSMU_HD inline long long num_need(long long S, long long n)
+{
+    return ((((12 * S + 7) & ~7LL) + 8 * n) + 15) & ~15LL;
+}
+
+// LDS kept back from the dynamic allocation for the kernels' static __shared__ arrays
+constexpr int LDS_STATIC_RESERVE = 256;
+
+// ---- launchers (kernels.hip); all enqueue on `s` and return the launch status ----
+
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+
+// B -> tiles: per-row tile counts, scan to tptr, fill tcol/tmask.
+// scan_ws must hold scan_ws_bytes(K) bytes.
+size_t scan_ws_bytes(int64_t n);
+hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const int *col,
+                         int *tcnt, int *tptr, int *tcol, unsigned long long *tmask, void *scan_ws,
+                         DevStats *st, int num_cus, hipStream_t s);
+
+// Per row of A: products, tile-products, tile span; totals into st.
+hipError_t launch_row_info(int64_t M, int64_t K, int64_t nnzA, const int *Arp, const int *Acol,
+                           const int *Brp, const int *tptr, const int *tcol, long long *rprod,
+                           int *rtp, int *rmin, int *rmax, int *rnnz, DevStats *st, int avg_a,
+                           hipStream_t s);
+
+// Bin rows for the symbolic (numeric = 0) or numeric (numeric = 1) phase.
+hipError_t launch_bin(int numeric, int64_t M, const int *Arp, const long long *rprod,
+                      const int *rtp, const int *rmin, const int *rmax, const int *rnnz,
+                      BinParams bp, int *binrows, DevStats *st, hipStream_t s);
+
+struct Operands {
+    const int *Arp, *Acol;
+    const double *Aval;
+    const int *Brp, *Bcol;
+    const double *Bval;
+    const int *tptr, *tcol;
+    const unsigned long long *tmask;
+    const int *rmin, *rmax;
+};
+
+// Global-memory workspace (bin BIN_GLOBAL): gws_blocks regions of gws_stride u64 each;
+// a region holds gws_tiles 64-bit masks followed by gws_tiles 32-bit offsets, with
+// gws_tiles >= the widest tile span in the bin.
+hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int count,
+                           int max_need, const Operands &op, int *rnnz, int tshift,
+                           unsigned long long *gws, long long gws_stride, int num_cus,
+                           int gws_blocks, hipStream_t s);
+
+hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *scan_ws,
+                               long long *total_copy, hipStream_t s);
+
+hipError_t launch_numeric(int bin, const int *rows, const int *count_dev, int count, int max_need,
+                          const Operands &op, const long long *crp, int *Ccol, double *Cval,
+                          int bshift, int tshift, unsigned long long *gws, long long gws_stride,
+                          long long gws_tiles, int gws_blocks, int *err, hipStream_t s);
+
+hipError_t launch_rowptr_i64_to_i32(const long long *in, int64_t n, int *out, hipStream_t s);
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cus, hipStream_t s);
+
+// Opt every kernel in to the device's full LDS size; call once per device.
+hipError_t configure_kernels(int lds_max);
+
+} // namespace hsmu

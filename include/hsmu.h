@@ -1,0 +1,231 @@
+/*
+ * hsmu.h -- C ABI of the MI355X-native HSMU SpGEMM library (libhsmu.so).
+ *
+ * C = A * B with A, B and C in CSR, on one AMD Instinct MI355X (gfx950).
+ * Plain pointers and sizes only: no HIP, torch or CUDA types cross this boundary
+ * (streams are passed as void*).
+ *
+ * Reference interfaces each entry point replaces (wuminqaq/HSMU-SpGEMM):
+ *   hsmu_spgemm          <- NHC_spgemm            src/NHC_spgemm.cuh:7-43
+ *                           (compressed_NHC_spgemm src/compressed/compressed_NHC_spgemm.cu:22-248,
+ *                            small_scale_NHC_spgemm src/small/small_scale_NHC_spgemm.cu:4-121)
+ *   hsmu_btiles          <- Initialize_compressed_mask64_matrix
+ *                                                 src/compressed/compressed_mask64.cuh:343-424
+ *   hsmu_csr_free        <- free_C_matrix         src/loading.cuh:152-166
+ *   hsmu_tiles_free      <- realse_NHC_tile       src/loading.cuh:95-104
+ *   hsmu_copy_h2d        <- HtD_value_col_ptr     src/loading.cuh:86-94 (one array at a time)
+ *   hsmu_read_mtx        <- mmio_allinone         src/mmio_highlevel.h:18-193
+ *   hsmu_transpose       <- csr2csc               src/mmio_highlevel.h:194-229
+ *   hsmu_ctx_create/destroy <- setup / realse_compressed_bin / free_bin
+ *                                                 src/loading.cuh:51-84,105-127,167-174
+ * The reference-compatible entry point NHC_spgemm itself is declared in NHC_spgemm.h.
+ *
+ * Semantics kept from the reference:
+ *   - rows of B must be sorted by column (the reference's numeric binary search,
+ *     src/compressed/compressed_Cval.cuh:55-61, needs it too); an unsorted row is an
+ *     error here (HSMU_ERR_UNSORTED) instead of an out-of-bounds access;
+ *   - columns of every row of C are sorted ascending;
+ *   - structural nonzeros are kept even when their value cancels to zero;
+ *   - repeated column indices in a row of B are summed.
+ * Extensions: int64 row pointers for C (nnz(C) >= 2^31), a compensated-FP32 numeric
+ * form, and a batched entry point.
+ */
+#ifndef HSMU_H
+#define HSMU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+enum hsmu_status {
+    HSMU_OK = 0,
+    HSMU_ERR_INVALID = 1,     /* bad argument: null pointer, negative size, A.N != B.M, ... */
+    HSMU_ERR_UNSORTED = 2,    /* a row of B has decreasing column indices */
+    HSMU_ERR_INDEX = 3,       /* a column index or row pointer out of range */
+    HSMU_ERR_OVERFLOW = 4,    /* nnz(C) does not fit the requested row-pointer type */
+    HSMU_ERR_NOMEM = 5,       /* device or host allocation failed */
+    HSMU_ERR_HIP = 6,         /* HIP runtime error (message in hsmu_last_error) */
+    HSMU_ERR_UNSUPPORTED = 7  /* valid request this build does not implement */
+};
+
+/* numeric form */
+enum hsmu_precision {
+    HSMU_FP64 = 0,      /* plain: FP64 products, FP64 LDS accumulation (the reference's value_t) */
+    HSMU_FP32_COMP = 1  /* FP32 products, compensated (two-sum) FP32 accumulation, FP32 values */
+};
+
+/* ---- matrices ---------------------------------------------------------------- */
+
+/* A CSR matrix in device memory.
+ * Inputs: rowptr is int32 (rowptr64 = 0) and val is double, or float when val32 = 1
+ * (FP32_COMP inputs).  Output C: rowptr is int32 or int64 (hsmu_opts.rowptr64), col int32,
+ * val double (FP64) or float (FP32_COMP). */
+typedef struct hsmu_csr {
+    int64_t M;        /* rows */
+    int64_t N;        /* columns */
+    int64_t nnz;      /* stored entries */
+    void *rowptr;     /* device [M+1], int32 or int64 */
+    int32_t *col;     /* device [nnz] */
+    void *val;        /* device [nnz], double or float */
+    int32_t rowptr64; /* 1: rowptr is int64 */
+    int32_t val32;    /* 1: val is float */
+} hsmu_csr;
+
+/* B in the compressed 64-column tile format of src/compressed/compressed_mask64.cuh:
+ * for row j, tiles tptr[j] .. tptr[j+1]-1; tile t covers columns 64*tcol[t] ..
+ * 64*tcol[t]+63 and bit b of tmask[t] is set when column 64*tcol[t]+b is stored.
+ * Unlike the reference (whose tile order comes from LDS atomics), the tiles of a row
+ * are sorted by tcol. */
+typedef struct hsmu_tiles {
+    int64_t M;        /* rows of B */
+    int64_t ntiles;   /* total tiles */
+    int32_t *tptr;    /* device [M+1] */
+    int32_t *tcol;    /* device [ntiles] */
+    uint64_t *tmask;  /* device [ntiles] */
+} hsmu_tiles;
+
+/* A CSR matrix in host memory (generators, Matrix Market reader).  Arrays are
+ * malloc'd by the library and released with hsmu_host_csr_free. */
+typedef struct hsmu_host_csr {
+    int64_t M, N, nnz;
+    int32_t *rowptr;  /* [M+1] */
+    int32_t *col;     /* [nnz] */
+    double *val;      /* [nnz] */
+} hsmu_host_csr;
+
+/* ---- options and statistics ---------------------------------------------------- */
+
+typedef struct hsmu_opts {
+    int32_t precision;        /* enum hsmu_precision */
+    int32_t rowptr64;         /* 1: C row pointers are int64 */
+    int32_t plain_alloc;      /* 1: allocate C with hipMalloc (release with hipFree, as the
+                                 reference's free_C_matrix does); 0: from the context pool */
+    int32_t force_global;     /* 1: every nonempty row takes the global-memory accumulator
+                                 (exercises the fallback on small inputs) */
+    int32_t lds_row_max;      /* bytes of LDS one row may use before it falls back to the
+                                 global-memory path; 0 = device maximum */
+    int32_t lds_block_budget; /* LDS bytes per workgroup for bins that pack several rows
+                                 into one workgroup; 0 = default (64 KiB) */
+    int32_t products_per_lane;/* numeric work per lane that sets a row's lane count; 0 = 8 */
+    int32_t tiles_per_lane;   /* symbolic work per lane that sets a row's lane count; 0 = 4 */
+    int32_t timing;           /* 1: record per-stage hipEvents into hsmu_stats */
+    int32_t b_lanes;          /* lanes that walk one row of B in the numeric phase (power of
+                                 two, 1..64); 0 = from the mean row length of B */
+    int32_t t_lanes;          /* lanes that walk one row of B's tiles in the tile OR (power
+                                 of two, 1..64); 0 = from the mean tiles per row of B */
+    int32_t reserved[5];
+} hsmu_opts;
+
+#define HSMU_NBINS 10 /* 8 LDS bins (8..1024 lanes per row), global fallback, wide rows */
+
+typedef struct hsmu_stats {
+    /* stage times in ms (hipEvents on the library stream), named after the
+     * reference's stages (src/compressed/compressed_NHC_spgemm.cu:36-170) */
+    double ms_conversion;     /* B -> tiles: Initialize_compressed_mask64_matrix */
+    double ms_form_tileCptr;  /* row analysis and binning: compressed_Form_tileCptr */
+    double ms_tileOR;         /* symbolic tile OR, nnz per row, C row pointers */
+    double ms_malloc;         /* allocation of C */
+    double ms_get_Ccol;       /* column extraction; fused into the numeric kernels (0) */
+    double ms_form_cval;      /* numeric: accumulate, write columns and values */
+    double ms_time2;          /* the reference's time2: all of the above but conversion */
+    double ms_total;          /* conversion + time2 */
+    double ms_numeric_bin[HSMU_NBINS]; /* per numeric bin kernel time (timing = 1) */
+    int64_t products;         /* intermediate products, sum over a_ij of nnz(B_j) */
+    int64_t tile_products;    /* sum over a_ij of tiles(B_j) */
+    int64_t nnz_c;
+    int64_t tiles_b;
+    int64_t rows_nonempty;
+    int64_t sym_bin_rows[HSMU_NBINS];
+    int64_t num_bin_rows[HSMU_NBINS];
+    double compress_rate;     /* products / nnz(C) (floating, the reference divides integers) */
+    double gflops;            /* 2 * products / time2 */
+    double space_compressed_mask_mb; /* size of B's tile format, MiB */
+    double alg_bytes;         /* algorithmic HBM bytes of the whole SpGEMM (gather model) */
+    double num_bin_bytes[HSMU_NBINS];    /* the same model over each numeric bin's rows */
+    int64_t num_bin_products[HSMU_NBINS];
+} hsmu_stats;
+
+/* ---- context ------------------------------------------------------------------- */
+
+typedef struct hsmu_ctx hsmu_ctx;
+
+/* Create a context on a device: one HIP stream (default priority), a caching device
+ * allocator, pinned staging memory.  One context per host thread. */
+int hsmu_ctx_create(int device, hsmu_ctx **out);
+void hsmu_ctx_destroy(hsmu_ctx *ctx);
+void *hsmu_ctx_stream(hsmu_ctx *ctx);       /* the hipStream_t, as void* */
+int hsmu_ctx_trim(hsmu_ctx *ctx);           /* release cached device memory */
+int64_t hsmu_ctx_pool_bytes(hsmu_ctx *ctx); /* device bytes held by the pool */
+int hsmu_ctx_lds_max(hsmu_ctx *ctx);        /* LDS bytes per workgroup on this device */
+int hsmu_ctx_num_cus(hsmu_ctx *ctx);
+
+void hsmu_opts_default(hsmu_opts *opts);
+const char *hsmu_status_string(int status);
+const char *hsmu_last_error(void);          /* message of the last failure on this thread */
+
+/* ---- the operator ---------------------------------------------------------------- */
+
+/* C = A * B.  A and B in device memory (int32 row pointers); the library allocates C's
+ * arrays (from the pool unless opts->plain_alloc) and fills every field of *C.
+ * Synchronous on return.  opts and stats may be NULL. */
+int hsmu_spgemm(hsmu_ctx *ctx, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C,
+                const hsmu_opts *opts, hsmu_stats *stats);
+
+/* Release C's device arrays (pool or hipFree, as allocated) and zero the struct. */
+int hsmu_csr_free(hsmu_ctx *ctx, hsmu_csr *C);
+
+/* B -> compressed 64-column tiles (the conversion stage on its own). */
+int hsmu_btiles(hsmu_ctx *ctx, const hsmu_csr *B, hsmu_tiles *T);
+int hsmu_tiles_free(hsmu_ctx *ctx, hsmu_tiles *T);
+
+/* ---- device memory helpers (for bindings that carry no device allocator) -------- */
+int hsmu_device_alloc(hsmu_ctx *ctx, size_t bytes, void **ptr);
+int hsmu_device_free(hsmu_ctx *ctx, void *ptr);
+int hsmu_copy_h2d(hsmu_ctx *ctx, void *dst, const void *src, size_t bytes);
+int hsmu_copy_d2h(hsmu_ctx *ctx, void *dst, const void *src, size_t bytes);
+int hsmu_sync(hsmu_ctx *ctx);
+
+/* Bandwidth probe: copy n bytes device to device with a plain vector copy kernel and
+ * return its time in ms (used to scale HBM counters and as the achievable-bandwidth
+ * figure beside the roofline). */
+int hsmu_copy_kernel(hsmu_ctx *ctx, void *dst, const void *src, size_t bytes, double *ms);
+
+/* ---- host-side inputs ------------------------------------------------------------ */
+
+/* 2D 5-point Laplacian on an n x n grid: row i = y*n + x, diagonal 4, neighbours -1. */
+int hsmu_gen_laplace2d(int32_t n, hsmu_host_csr *out);
+/* 3D 7-point Laplacian on an n^3 grid: diagonal 6, neighbours -1. */
+int hsmu_gen_laplace3d(int32_t n, hsmu_host_csr *out);
+/* Banded FEM-like: M x M, per-row nnz uniform in [lo, hi], distinct columns uniform in
+ * [i - halfwidth, i + halfwidth] clipped to [0, M), values N(0,1). */
+int hsmu_gen_banded(int32_t M, int32_t lo, int32_t hi, int32_t halfwidth, uint64_t seed,
+                    hsmu_host_csr *out);
+/* R-MAT: 2^scale x 2^scale, edge_factor * 2^scale generated edges with quadrant
+ * probabilities a, b, c (d = 1 - a - b - c), duplicates merged, values U(0,1). */
+int hsmu_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                  uint64_t seed, hsmu_host_csr *out);
+/* Zipf: M x N, per_row distinct columns per row drawn Zipf(s) over the columns,
+ * values U(-1,1). */
+int hsmu_gen_zipf(int32_t M, int32_t N, int32_t per_row, double s, uint64_t seed,
+                  hsmu_host_csr *out);
+/* Random sparse M x N with per-row nnz uniform in [0, max_per_row], sorted distinct
+ * columns, values U(-1,1) (test inputs). */
+int hsmu_gen_random(int32_t M, int32_t N, int32_t max_per_row, uint64_t seed,
+                    hsmu_host_csr *out);
+/* Transpose (CSR -> CSR of A^T, i.e. csr2csc); rows of the result are sorted. */
+int hsmu_transpose(const hsmu_host_csr *A, hsmu_host_csr *AT);
+/* Matrix Market coordinate file -> CSR with the reference loader's semantics
+ * (symmetric expansion, pattern values 1.0, duplicates kept); rows are then sorted
+ * by column (stable), which the reference leaves to the file. */
+int hsmu_read_mtx(const char *path, hsmu_host_csr *out, int32_t *is_symmetric);
+void hsmu_host_csr_free(hsmu_host_csr *m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HSMU_H */

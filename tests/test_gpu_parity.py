@@ -1,0 +1,281 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Structure (row pointers, columns) must be bit-exact; FP64 values must satisfy
+|c - c_ref| <= 1e-12 * sum_j |a_ij * b_jk| (the tolerance of the north-star contract;
+LDS/L2 atomics make the summation order nondeterministic, as in the reference).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import hsmu
+import oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def note(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def check(ctx, A, B, name, tol=TOL, **opts):
+    note(f"{name}: A {A.M}x{A.N} nnz {A.nnz}, B {B.M}x{B.N} nnz {B.nnz}, opts {opts}")
+    Cm, st = ctx.spgemm(A, B, **opts)
+    ref = oracle.spgemm(A, B)
+    summary = oracle.compare(Cm, ref, tol=tol)
+    assert st["nnz_c"] == Cm.nnz == int(ref.rowptr[-1])
+    assert st["products"] == oracle.products(A, B)
+    return Cm, st, summary
+
+
+def test_golden_test_mtx(ctx):
+    import json
+    A, _ = hsmu.read_mtx(os.path.join(GOLDEN, "test.mtx"))
+    g = json.load(open(os.path.join(GOLDEN, "test_mtx_AxA.json")))
+    Cm, st, _ = check(ctx, A, A, "test.mtx")
+    assert Cm.rowptr.tolist() == g["rowptr"] and Cm.col.tolist() == g["col"]
+    assert Cm.val.tolist() == g["val"]  # small integers: exact
+    assert st["products"] == 19
+
+
+def test_laplace2d_128(ctx):
+    A = hsmu.gen_laplace2d(128)
+    Cm, st, _ = check(ctx, A, A, "laplace2d 128")
+    assert Cm.nnz == 210436 and st["products"] == 405000
+    ref = oracle.spgemm(A, A)
+    assert np.array_equal(Cm.val, ref.val)  # integer-valued: exact
+
+
+@pytest.mark.parametrize("n", [2, 9, 24])
+def test_laplace3d(ctx, n):
+    A = hsmu.gen_laplace3d(n)
+    Cm, _, _ = check(ctx, A, A, f"laplace3d {n}")
+    assert np.array_equal(Cm.val, oracle.spgemm(A, A).val)
+
+
+@pytest.mark.parametrize("M,lo,hi,hw", [(3000, 40, 88, 400), (20000, 5, 30, 2000), (9000, 100, 200, 150)])
+def test_banded_small(ctx, M, lo, hi, hw):
+    A = hsmu.gen_banded(M, lo, hi, hw, seed=M)
+    check(ctx, A, A, f"banded {M}")
+
+
+def test_banded_full(ctx):
+    A = hsmu.gen_banded()
+    Cm, st, s = check(ctx, A, A, "banded 62451 (full config)")
+    note(f"banded full: nnz {Cm.nnz} products {st['products']} max_rel {s['max_rel_err']:.2e}")
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("shape", [(50, 70, 90), (300, 200, 5000), (1000, 3000, 70000), (7, 1, 300)])
+def test_random_rectangular(ctx, shape, seed):
+    m, k, n = shape
+    A = hsmu.gen_random(m, k, 12, seed)
+    B = hsmu.gen_random(k, n, 40, seed + 100)
+    check(ctx, A, B, f"random {shape} seed {seed}")
+
+
+def test_small_n_path(ctx):
+    """B.N < 4096: the reference's small (dense bitmap) path; same kernels here."""
+    A = hsmu.gen_random(2000, 1500, 30, 11)
+    B = hsmu.gen_random(1500, 1000, 60, 12)
+    check(ctx, A, B, "small N")
+
+
+def test_empty_rows_and_columns(ctx):
+    A = hsmu.gen_random(500, 400, 3, 5)  # many empty rows
+    B = hsmu.gen_random(400, 9000, 2, 6)
+    check(ctx, A, B, "sparse with empty rows")
+
+
+@pytest.mark.parametrize("case", ["A_empty", "B_empty", "zero_rows"])
+def test_empty_inputs(ctx, case):
+    if case == "A_empty":
+        A = hsmu.HostMatrix(10, 20, np.zeros(11, np.int32), np.zeros(0, np.int32), np.zeros(0))
+        B = hsmu.gen_random(20, 30, 5, 1)
+    elif case == "B_empty":
+        A = hsmu.gen_random(10, 20, 5, 1)
+        B = hsmu.HostMatrix(20, 30, np.zeros(21, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    else:
+        A = hsmu.HostMatrix(0, 20, np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+        B = hsmu.gen_random(20, 30, 5, 1)
+    note(f"empty input {case}")
+    Cm, st = ctx.spgemm(A, B)
+    assert Cm.nnz == 0 and Cm.M == A.M and np.all(Cm.rowptr == 0)
+
+
+def test_duplicates_in_B_are_summed(ctx):
+    B = hsmu.gen_random(300, 2000, 20, 3)
+    rp, col, val = [0], [], []
+    for i in range(B.M):
+        c = B.col[B.rowptr[i]:B.rowptr[i + 1]]
+        v = B.val[B.rowptr[i]:B.rowptr[i + 1]]
+        c2 = np.sort(np.concatenate([c, c[::3]]), kind="stable")
+        v2 = np.concatenate([v, v[::3]])[np.argsort(np.concatenate([c, c[::3]]), kind="stable")]
+        col.extend(c2.tolist())
+        val.extend(v2.tolist())
+        rp.append(len(col))
+    Bd = hsmu.HostMatrix(B.M, B.N, np.array(rp, np.int32), np.array(col, np.int32), np.array(val))
+    A = hsmu.gen_random(200, 300, 15, 4)
+    check(ctx, A, Bd, "duplicates in B")
+
+
+def test_cancellation_keeps_structure(ctx):
+    """Structural nonzeros stay even when the value cancels to exactly zero."""
+    A = hsmu.HostMatrix(1, 2, np.array([0, 2], np.int32), np.array([0, 1], np.int32), np.array([1.0, -1.0]))
+    B = hsmu.HostMatrix(2, 3, np.array([0, 2, 4], np.int32), np.array([0, 2, 0, 2], np.int32),
+                        np.array([2.0, 5.0, 2.0, 5.0]))
+    Cm, _, _ = check(ctx, A, B, "cancellation")
+    assert Cm.col.tolist() == [0, 2] and Cm.val.tolist() == [0.0, 0.0]
+
+
+@pytest.mark.parametrize("opts", [dict(force_global=1), dict(lds_row_max=256), dict(lds_block_budget=4096),
+                                  dict(products_per_lane=1), dict(products_per_lane=512, tiles_per_lane=256),
+                                  dict(b_lanes=1, t_lanes=1), dict(b_lanes=64, t_lanes=64)])
+def test_bins_and_fallbacks(ctx, opts):
+    A = hsmu.gen_banded(4000, 10, 120, 600, seed=7)
+    check(ctx, A, A, "banded under forced bins", **opts)
+    B = hsmu.gen_random(800, 20000, 50, 8)
+    A2 = hsmu.gen_random(600, 800, 40, 9)
+    check(ctx, A2, B, "random under forced bins", **opts)
+
+
+def test_rowptr64(ctx):
+    A = hsmu.gen_banded(5000, 20, 60, 300, seed=2)
+    Cm, _, _ = check(ctx, A, A, "int64 row pointers", rowptr64=1)
+    assert Cm.rowptr.dtype == np.int64
+
+
+@pytest.mark.parametrize("scale", [10, 13])
+def test_rmat(ctx, scale):
+    A = hsmu.gen_rmat(scale, 16, seed=2)
+    check(ctx, A, A, f"rmat scale {scale}")
+
+
+def test_zipf_aat(ctx):
+    A = hsmu.gen_zipf(4000, 2000, 8, 1.2, seed=4)
+    check(ctx, A, hsmu.transpose(A), "zipf A*A^T")
+
+
+def test_btiles_match_oracle(ctx):
+    for name, B in [("banded", hsmu.gen_banded(3000, 40, 88, 400, seed=1)),
+                    ("random", hsmu.gen_random(700, 50000, 90, 3)), ("laplace", hsmu.gen_laplace2d(40))]:
+        note(f"btiles {name}")
+        tptr, tcol, tmask = ctx.btiles(B)
+        rp, rc, rm = oracle.btiles(B)
+        assert np.array_equal(tptr, rp) and np.array_equal(tcol, rc) and np.array_equal(tmask, rm)
+
+
+def test_errors_fail_loudly(ctx):
+    A = hsmu.gen_random(50, 60, 5, 1)
+    B = hsmu.gen_random(60, 70, 5, 2)
+    note("error: shape mismatch")
+    with pytest.raises(hsmu.HsmuError) as e:
+        ctx.spgemm(A, hsmu.gen_random(61, 70, 5, 2))
+    assert e.value.status == hsmu.HSMU_ERR_INVALID
+    note("error: unsorted B")
+    Bu = hsmu.HostMatrix(B.M, B.N, B.rowptr, B.col.copy(), B.val)
+    r = int(np.argmax(np.diff(B.rowptr) >= 2))
+    s = Bu.rowptr[r]
+    Bu.col[s], Bu.col[s + 1] = Bu.col[s + 1], Bu.col[s]
+    with pytest.raises(hsmu.HsmuError) as e:
+        ctx.spgemm(A, Bu)
+    assert e.value.status == hsmu.HSMU_ERR_UNSORTED
+    note("error: column out of range")
+    Ab = hsmu.HostMatrix(A.M, A.N, A.rowptr, A.col.copy(), A.val)
+    Ab.col[0] = 60
+    with pytest.raises(hsmu.HsmuError) as e:
+        ctx.spgemm(Ab, B)
+    assert e.value.status == hsmu.HSMU_ERR_INDEX
+    note("after errors the context still works")
+    check(ctx, A, B, "after errors")
+
+
+def test_repeated_calls_reuse_pool(ctx):
+    A = hsmu.gen_banded(6000, 30, 60, 500, seed=3)
+    dA = ctx.matrix(A)
+    ref = oracle.spgemm(A, A)
+    held = None
+    for it in range(4):
+        note(f"repeat {it}")
+        c, st = ctx.spgemm_device(dA, dA)
+        oracle.compare(ctx.fetch(c), ref)
+        ctx.free_result(c)
+        pool = hsmu.lib.hsmu_ctx_pool_bytes(ctx.handle)
+        if held is not None:
+            assert pool == held  # steady state: no new device allocations
+        held = pool
+    dA.free()
+
+
+class NHC_CSR(ctypes.Structure):
+    _fields_ = [("rowPtr", ctypes.c_void_p), ("col", ctypes.c_void_p), ("val", ctypes.c_void_p),
+                ("d_ptr", ctypes.c_void_p), ("d_col", ctypes.c_void_p), ("d_val", ctypes.c_void_p),
+                ("d_tem_ptr", ctypes.c_void_p), ("M", ctypes.c_int), ("N", ctypes.c_int), ("nnz", ctypes.c_int),
+                ("nnz_max", ctypes.c_int), ("isSymmetric", ctypes.c_int), ("matrix_name", ctypes.c_char_p),
+                ("d_tile_ptr", ctypes.c_void_p), ("d_tile_col", ctypes.c_void_p),
+                ("d_tem_tile_ptr", ctypes.c_void_p), ("tile_ptr", ctypes.c_void_p),
+                ("d_mask_num", ctypes.c_void_p)]
+
+
+def test_nhc_spgemm_dropin(ctx):
+    """The reference-signature entry point through the C ABI."""
+    lib = hsmu.lib
+    A = hsmu.gen_banded(4000, 30, 60, 900, seed=5)
+    keep = []
+
+    def nhc(m):
+        x = NHC_CSR()
+        x.M, x.N, x.nnz = m.M, m.N, m.nnz
+        for f, arr in (("rowPtr", m.rowptr.astype(np.int32)), ("col", m.col.astype(np.int32)),
+                       ("val", m.val.astype(np.float64))):
+            arr = np.ascontiguousarray(arr)
+            keep.append(arr)
+            setattr(x, f, arr.ctypes.data)
+        lib.HtD_value_col_ptr(ctypes.byref(x))
+        return x
+
+    a, b = nhc(A), nhc(A)
+    c = NHC_CSR()
+    c.M, c.N = A.M, A.N
+    cb = (ctypes.c_char * 136)()
+    bn = (ctypes.c_char * 32)()
+    mk = (ctypes.c_char * 88)()
+    note("NHC_spgemm banded 4000")
+    lib.NHC_spgemm(cb, bn, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), mk)
+    assert lib.NHC_last_status() == 0
+    rp = np.ctypeslib.as_array(ctypes.cast(c.rowPtr, ctypes.POINTER(ctypes.c_int)), shape=(A.M + 1,)).copy()
+    col = ctx.download(c.d_col, c.nnz, np.int32)
+    val = ctx.download(c.d_val, c.nnz, np.float64)
+    oracle.compare(hsmu.HostMatrix(A.M, A.N, rp, col, val), oracle.spgemm(A, A))
+    lib.free_C_matrix(ctypes.byref(c))
+    for x in (a, b):  # device arrays only: host arrays belong to numpy here
+        hsmu.lib.hsmu_device_free(ctx.handle, x.d_ptr)
+        hsmu.lib.hsmu_device_free(ctx.handle, x.d_col)
+        hsmu.lib.hsmu_device_free(ctx.handle, x.d_val)
+
+
+def test_driver_prints_reference_lines(tmp_path):
+    exe = os.path.join(os.path.dirname(hsmu.LIB_PATH), "hsmu_test")
+    note("driver on test.mtx")
+    out = subprocess.run([exe, os.path.join(GOLDEN, "test.mtx"), "--csv-dir", str(tmp_path)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    text = out.stdout
+    for line in ["the matrix is: --------------", "success open file", "input matrix A: ( 5, 5 ) nnz = 9",
+                 "nums_Intermediate_product = 19", "cnnz = 11", "the total run time of NHC_spgemm is"]:
+        assert line in text, (line, text)
+    note("driver on gen:laplace2d:128")
+    out = subprocess.run([exe, "gen:laplace2d:128"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    for line in ["the run time for Initialize_compressed_mask64_matrix is", "space_compressed_mask =",
+                 "the run time for compressed_Form_tileCptr is", "the run time for h_compress_tileOR_for_Cptr is",
+                 "the run time for cudaMalloc col and val is", "the run time for h_get_Ccol_from_Ctile is",
+                 "the run time for h_form_cval is", "nums_Intermediate_product = 405000", "cnnz = 210436",
+                 "compress_rate =", "the gflops is"]:
+        assert line in out.stdout, (line, out.stdout)
+    assert os.path.exists(tmp_path / "NHC_MI355X_result.csv")
