@@ -41,6 +41,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--opts", default="", help="hsmu_opts overrides, e.g. products_per_lane=16,b_lanes=32")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -54,7 +55,8 @@ def main():
     log(f"generated banded A: {A.M} rows, nnz {A.nnz} in {time.time() - t0:.1f}s")
     ctx = hsmu.Context(0)
     dA = ctx.matrix(A)
-    opts = hsmu.default_opts()
+    overrides = dict(kv.split("=") for kv in args.opts.split(",") if kv)
+    opts = hsmu.default_opts(**{k: int(v) for k, v in overrides.items()})
 
     for _ in range(args.warmup):
         c, st = ctx.spgemm_device(dA, dA, opts)

@@ -1,0 +1,489 @@
+// accum.hip -- symbolic and numeric row kernels of the HSMU SpGEMM path for gfx950.
+//
+// Symbolic: nnz(C_i) from the OR of B's 64-column tile masks over row i of A, in a dense
+// mask directory over the row's tile span (reference: compressed_tilePtr_symbolic.cuh
+// and compressed_tileOR.cuh, two hash passes over tiles there; one pass and no hashing
+// here).  Numeric: the HSMU accumulator -- exactly nnz(C_i) FP64 slots in LDS, the slot
+// of column c being off[t] + popcount(mask[t] & below(c & 63)), t = (c >> 6) - rmin,
+// i.e. c's rank among C's sorted columns found in O(1) (reference: binary search over
+// the sorted column list, src/compressed/compressed_Cval.cuh:5-69 and
+// src/small/numeric.cuh:233-253).  The numeric kernel also writes C's sorted columns,
+// replacing the reference's rank-sort pass (compressed_Ccol_symbolic.cuh:4-63).
+//
+// LDS kernels first stage the row's A entries in LDS (B row start, tile start, a_ij, and
+// prefix sums of nnz(B_j) and tiles(B_j)), then walk the row's products (or
+// tile-products) as one flat sequence: each wave takes 64 consecutive items, so lanes
+// load consecutive elements of B (coalesced) and no lane waits on a per-entry chain of
+// dependent loads.  Rows whose slice does not fit LDS use the global-memory kernels.
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "device_common.h"
+#include "hsmu_internal.h"
+
+namespace hsmu {
+
+// Stage the row's A entries a0 .. a0+nA-1 into LDS: first tile and tile count of B_j,
+// and (numeric) first element and length of B_j and a_ij.  One parallel step per row,
+// so the dependent loads Acol -> tptr/Brp are paid once per row rather than once per
+// entry inside the accumulation loops.
+template <int G, bool NUMERIC>
+DEVI void stage_row(const Operands &op, int a0, int nA, int *t0, int *tlen, int *b0, int *blen, double *av)
+{
+    const int l = threadIdx.x & (G - 1);
+    for (int k = l; k < nA; k += G) {
+        const int j = op.Acol[a0 + k];
+        const int tt = op.tptr[j];
+        t0[k] = tt;
+        tlen[k] = op.tptr[j + 1] - tt;
+        if constexpr (NUMERIC) {
+            const int bb = op.Brp[j];
+            b0[k] = bb;
+            blen[k] = op.Brp[j + 1] - bb;
+            av[k] = op.Aval[a0 + k];
+        }
+    }
+}
+
+constexpr int UNROLL = 4;
+
+// OR the tiles of B_j into dm for every entry j of the row: sub-groups of 2^tshift
+// lanes take one entry each and walk its tiles, UNROLL tiles per lane per iteration
+// with all loads issued before the ds_or_b64s.
+template <int G>
+DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, u64 *dm, int mn, int tshift)
+{
+    const int l = threadIdx.x & (G - 1);
+    const int SG = 1 << tshift, sl = l & (SG - 1), NSG = G >> tshift;
+    for (int e = l >> tshift; e < nA; e += NSG) {
+        const int tb = t0[e], te = tb + tlen[e];
+        for (int t = tb + sl; t < te; t += UNROLL * SG) {
+            int c[UNROLL];
+            u64 m[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                const int tu = t + u * SG;
+                c[u] = -1;
+                m[u] = 0;
+                if (tu < te) {
+                    c[u] = op.tcol[tu];
+                    m[u] = op.tmask[tu];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+                if (c[u] >= 0)
+                    atomicOr(&dm[c[u] - mn], m[u]);
+        }
+    }
+}
+
+// Add every product a_ij * b_jc of the row into its slot: sub-groups of 2^bshift lanes
+// take one entry each and walk B_j, UNROLL elements per lane per iteration with all
+// loads issued before the ds_add_f64s.
+template <int G>
+DEVI void add_products(const Operands &op, const int *b0, const int *blen, const double *av, int nA, const u64 *dm,
+                       const unsigned *doff, double *vals, int mn, int bshift)
+{
+    const int l = threadIdx.x & (G - 1);
+    const int SG = 1 << bshift, sl = l & (SG - 1), NSG = G >> bshift;
+    for (int e = l >> bshift; e < nA; e += NSG) {
+        const int kb = b0[e], ke = kb + blen[e];
+        const double a = av[e];
+        for (int k = kb + sl; k < ke; k += UNROLL * SG) {
+            int c[UNROLL];
+            double v[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                const int ku = k + u * SG;
+                c[u] = -1;
+                v[u] = 0.0;
+                if (ku < ke) {
+                    c[u] = op.Bcol[ku];
+                    v[u] = op.Bval[ku];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                if (c[u] >= 0) {
+                    const int t = (c[u] >> 6) - mn;
+                    const unsigned slot = doff[t] + (unsigned)__popcll(dm[t] & below(c[u] & 63));
+                    atomicAdd(&vals[slot], a * v[u]);
+                }
+            }
+        }
+    }
+}
+
+// ---- symbolic ------------------------------------------------------------------------------
+
+// nnz(C_i) for rows kept in LDS: stage the row, OR the tile masks with ds_or_b64 into a
+// dense directory over the row's tile span, popcount.
+template <int G>
+__global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const int *__restrict__ rows,
+                                                                       const int *__restrict__ count_p, int slice_bytes,
+                                                                       Operands op, int *__restrict__ rnnz, int tshift)
+{
+    constexpr int BS = G < BLOCK ? BLOCK : G;
+    constexpr int RPB = BS / G;
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int redi[16];
+    const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
+    unsigned char *slice = lds + (size_t)g * slice_bytes;
+    const int nrows = *count_p;
+    for (int r0 = xcd_block(blockIdx.x, gridDim.x) * RPB; r0 < nrows; r0 += gridDim.x * RPB) {
+        const int idx = r0 + g;
+        const int row = idx < nrows ? rows[idx] : -1;
+        int S = 0, mn = 0, a0 = 0, nA = 0;
+        if (row >= 0) {
+            mn = op.rmin[row];
+            S = op.rmax[row] - mn + 1;
+            a0 = op.Arp[row];
+            nA = op.Arp[row + 1] - a0;
+        }
+        const RowLayout L = row_layout(0, S, 0, nA);
+        u64 *dm = (u64 *)(slice + L.dm);
+        int *tlen = (int *)(slice + L.tlen);
+        int *t0 = (int *)(slice + L.t0);
+        for (int t = l; t < S; t += G)
+            dm[t] = 0;
+        stage_row<G, false>(op, a0, nA, t0, tlen, nullptr, nullptr, nullptr);
+        __syncthreads();
+        or_tiles<G>(op, t0, tlen, nA, dm, mn, tshift);
+        __syncthreads();
+        int cnt = 0;
+        for (int t = l; t < S; t += G)
+            cnt += __popcll(dm[t]);
+        cnt = group_sum<G>(cnt, redi);
+        if (row >= 0 && l == 0)
+            rnnz[row] = cnt;
+        __syncthreads();
+    }
+}
+
+// Rows too wide for LDS: the same dense directory in a per-workgroup global workspace,
+// updated with L2 atomics.  One 1024-lane workgroup per row.
+__global__ __launch_bounds__(1024) void k_symbolic_global(const int *__restrict__ rows, const int *__restrict__ count_p,
+                                                          Operands op, int *__restrict__ rnnz, int tshift, u64 *gws,
+                                                          long long gws_stride)
+{
+    __shared__ int red[16];
+    const int l = threadIdx.x;
+    u64 *dm = gws + (size_t)blockIdx.x * gws_stride;
+    const int nrows = *count_p;
+    const int SG = 1 << tshift, sl = l & (SG - 1), sg = l >> tshift, NSG = 1024 >> tshift;
+    for (int idx = blockIdx.x; idx < nrows; idx += gridDim.x) {
+        const int row = rows[idx];
+        const int mn = op.rmin[row];
+        const int S = op.rmax[row] - mn + 1;
+        for (int t = l; t < S; t += 1024)
+            dm[t] = 0;
+        block_sync_global();
+        const int a1 = op.Arp[row + 1];
+        for (int e = op.Arp[row] + sg; e < a1; e += NSG) {
+            const int j = op.Acol[e];
+            const int t1 = op.tptr[j + 1];
+            for (int t = op.tptr[j] + sl; t < t1; t += SG)
+                atomicOr(&dm[op.tcol[t] - mn], op.tmask[t]);
+        }
+        block_sync_global();
+        int cnt = 0;
+        for (int t = l; t < S; t += 1024)
+            cnt += __popcll(dm[t]);
+        cnt = group_sum<1024>(cnt, red);
+        if (l == 0)
+            rnnz[row] = cnt;
+        block_sync_global();
+    }
+}
+
+// ---- numeric -------------------------------------------------------------------------------
+
+// The HSMU accumulator in LDS (slice layout: row_layout).  After staging, the row's tile
+// directory is rebuilt as in the symbolic phase, an exclusive scan of its popcounts
+// gives every tile's first slot, every product is added with ds_add_f64 into its slot,
+// and the row of C is written in column order: for G >= 64 one wave per tile, lane b
+// writing column 64*(rmin+t)+b if present (coalesced); for G < 64 one lane per tile
+// walking its set bits.
+template <int G>
+__global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int *__restrict__ rows,
+                                                                      const int *__restrict__ count_p, int slice_bytes,
+                                                                      Operands op, const long long *__restrict__ crp,
+                                                                      int *__restrict__ Ccol, double *__restrict__ Cval,
+                                                                      int bshift, int tshift, int *err)
+{
+    constexpr int BS = G < BLOCK ? BLOCK : G;
+    constexpr int RPB = BS / G;
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned redu[16];
+    const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
+    unsigned char *slice = lds + (size_t)g * slice_bytes;
+    const int nrows = *count_p;
+    for (int r0 = xcd_block(blockIdx.x, gridDim.x) * RPB; r0 < nrows; r0 += gridDim.x * RPB) {
+        const int idx = r0 + g;
+        const int row = idx < nrows ? rows[idx] : -1;
+        int S = 0, mn = 0, n = 0, a0 = 0, nA = 0;
+        long long cb = 0;
+        if (row >= 0) {
+            mn = op.rmin[row];
+            S = op.rmax[row] - mn + 1;
+            cb = crp[row];
+            n = (int)(crp[row + 1] - cb);
+            a0 = op.Arp[row];
+            nA = op.Arp[row + 1] - a0;
+        }
+        const RowLayout L = row_layout(1, S, n, nA);
+        u64 *dm = (u64 *)(slice + L.dm);
+        unsigned *doff = (unsigned *)(slice + L.doff);
+        double *vals = (double *)(slice + L.vals);
+        double *av = (double *)(slice + L.av);
+        int *tlen = (int *)(slice + L.tlen);
+        int *t0 = (int *)(slice + L.t0);
+        int *blen = (int *)(slice + L.blen);
+        int *b0 = (int *)(slice + L.b0);
+        for (int t = l; t < S; t += G)
+            dm[t] = 0;
+        for (int q = l; q < n; q += G)
+            vals[q] = 0.0;
+        stage_row<G, true>(op, a0, nA, t0, tlen, b0, blen, av);
+        __syncthreads();
+        or_tiles<G>(op, t0, tlen, nA, dm, mn, tshift);
+        __syncthreads();
+        // slot offsets: each lane scans a contiguous chunk of tiles
+        const int chunk = (S + G - 1) / G;
+        const int ta = l * chunk;
+        const int tb = ta + chunk < S ? ta + chunk : S;
+        unsigned loc = 0;
+        for (int t = ta; t < tb; ++t)
+            loc += (unsigned)__popcll(dm[t]);
+        unsigned tot;
+        unsigned pre = group_excl_scan<G>(loc, redu, tot);
+        for (int t = ta; t < tb; ++t) {
+            doff[t] = pre;
+            pre += (unsigned)__popcll(dm[t]);
+        }
+        const bool ok = (int)tot == n;
+        if (row >= 0 && !ok && l == 0)
+            atomicOr(err, ERR_INTERNAL);
+        __syncthreads();
+        if (row >= 0 && ok)
+            add_products<G>(op, b0, blen, av, nA, dm, doff, vals, mn, bshift);
+        __syncthreads();
+        if (row >= 0 && ok) {
+            if constexpr (G >= 64) {
+                constexpr int NW = G / 64;
+                const int w = l >> 6, lane = l & 63;
+                for (int t = w; t < S; t += NW) {
+                    const u64 m = dm[t];
+                    if ((m >> lane) & 1ull) {
+                        const unsigned pos = doff[t] + (unsigned)__popcll(m & below(lane));
+                        Ccol[cb + pos] = ((mn + t) << 6) + lane;
+                        Cval[cb + pos] = vals[pos];
+                    }
+                }
+            } else {
+                for (int t = l; t < S; t += G) {
+                    u64 m = dm[t];
+                    unsigned pos = doff[t];
+                    const int c0 = (mn + t) << 6;
+                    while (m) {
+                        const int b = __ffsll((unsigned long long)m) - 1;
+                        Ccol[cb + pos] = c0 + b;
+                        ++pos;
+                        m &= m - 1;
+                    }
+                }
+                for (int q = l; q < n; q += G)
+                    Cval[cb + q] = vals[q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Rows that do not fit LDS: directory (masks + offsets) in a per-workgroup global
+// workspace, FP64 accumulation with global atomics straight into C's value array.
+__global__ __launch_bounds__(1024) void k_numeric_global(const int *__restrict__ rows, const int *__restrict__ count_p,
+                                                         Operands op, const long long *__restrict__ crp,
+                                                         int *__restrict__ Ccol, double *Cval, int bshift, int tshift,
+                                                         u64 *gws, long long gws_stride, long long gws_tiles, int *err)
+{
+    __shared__ unsigned red[16];
+    __shared__ int ok_s;
+    const int l = threadIdx.x;
+    // workgroup region: gws_tiles masks, then gws_tiles 32-bit offsets
+    u64 *dm = gws + (size_t)blockIdx.x * gws_stride;
+    unsigned *doff = (unsigned *)(dm + gws_tiles);
+    const int nrows = *count_p;
+    for (int idx = blockIdx.x; idx < nrows; idx += gridDim.x) {
+        const int row = rows[idx];
+        const int mn = op.rmin[row];
+        const int S = op.rmax[row] - mn + 1;
+        const long long cb = crp[row];
+        const int n = (int)(crp[row + 1] - cb);
+        for (int t = l; t < S; t += 1024)
+            dm[t] = 0;
+        for (int q = l; q < n; q += 1024)
+            Cval[cb + q] = 0.0;
+        block_sync_global();
+        {
+            const int SG = 1 << tshift, sl = l & (SG - 1), sg = l >> tshift, NSG = 1024 >> tshift;
+            const int a1 = op.Arp[row + 1];
+            for (int e = op.Arp[row] + sg; e < a1; e += NSG) {
+                const int j = op.Acol[e];
+                const int t1 = op.tptr[j + 1];
+                for (int t = op.tptr[j] + sl; t < t1; t += SG)
+                    atomicOr(&dm[op.tcol[t] - mn], op.tmask[t]);
+            }
+        }
+        block_sync_global();
+        const int chunk = (S + 1023) / 1024;
+        const int ta = l * chunk;
+        const int tb = ta + chunk < S ? ta + chunk : S;
+        unsigned loc = 0;
+        for (int t = ta; t < tb; ++t)
+            loc += (unsigned)__popcll(dm[t]);
+        unsigned tot;
+        unsigned pre = group_excl_scan<1024>(loc, red, tot);
+        for (int t = ta; t < tb; ++t) {
+            doff[t] = pre;
+            pre += (unsigned)__popcll(dm[t]);
+        }
+        if (l == 0) {
+            ok_s = (int)tot == n;
+            if ((int)tot != n)
+                atomicOr(err, ERR_INTERNAL);
+        }
+        block_sync_global();
+        if (ok_s) {
+            const int SG = 1 << bshift, sl = l & (SG - 1), sg = l >> bshift, NSG = 1024 >> bshift;
+            const int a1 = op.Arp[row + 1];
+            for (int e = op.Arp[row] + sg; e < a1; e += NSG) {
+                const int j = op.Acol[e];
+                const double a = op.Aval[e];
+                const int b1 = op.Brp[j + 1];
+                for (int k = op.Brp[j] + sl; k < b1; k += SG) {
+                    const int c = op.Bcol[k];
+                    const int t = (c >> 6) - mn;
+                    const unsigned slot = doff[t] + (unsigned)__popcll(dm[t] & below(c & 63));
+                    unsafeAtomicAdd(&Cval[cb + slot], a * op.Bval[k]);
+                }
+            }
+            const int w = l >> 6, lane = l & 63;
+            for (int t = w; t < S; t += 16) {
+                const u64 m = dm[t];
+                if ((m >> lane) & 1ull) {
+                    const unsigned pos = doff[t] + (unsigned)__popcll(m & below(lane));
+                    Ccol[cb + pos] = ((mn + t) << 6) + lane;
+                }
+            }
+        }
+        block_sync_global();
+    }
+}
+
+// ---- launchers -------------------------------------------------------------------------------
+
+template <int G>
+static void symbolic_g(const int *rows, const int *count_dev, int count, int max_need, const Operands &op, int *rnnz,
+                       int tshift, hipStream_t s)
+{
+    constexpr int BS = G < BLOCK ? BLOCK : G;
+    constexpr int RPB = BS / G;
+    const int slice = (max_need + 15) & ~15;
+    int64_t blocks = ((int64_t)count + RPB - 1) / RPB;
+    if (blocks > (1 << 20))
+        blocks = 1 << 20;
+    const int lg = __builtin_ctz(G);
+    hipLaunchKernelGGL((k_symbolic_lds<G>), dim3((unsigned)blocks), dim3(BS), (size_t)slice * RPB, s, rows, count_dev,
+                       slice, op, rnnz, tshift > lg ? lg : tshift);
+}
+
+hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
+                           int *rnnz, int tshift, u64 *gws, long long gws_stride, int num_cus, int gws_blocks,
+                           hipStream_t s)
+{
+    if (count <= 0)
+        return hipSuccess;
+    switch (bin) {
+    case 0: symbolic_g<8>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
+    case 1: symbolic_g<16>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
+    case 2: symbolic_g<32>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
+    case 3: symbolic_g<64>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
+    case 4: symbolic_g<128>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
+    case 5: symbolic_g<256>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
+    case 6: symbolic_g<512>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
+    case 7: symbolic_g<1024>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
+    case BIN_GLOBAL: {
+        const int blocks = count < gws_blocks ? count : gws_blocks;
+        hipLaunchKernelGGL(k_symbolic_global, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, rnnz,
+                           tshift > 10 ? 10 : tshift, gws, gws_stride);
+        break;
+    }
+    default: return hipErrorInvalidValue;
+    }
+    (void)num_cus;
+    return hipGetLastError();
+}
+
+template <int G>
+static void numeric_g(const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
+                      const long long *crp, int *Ccol, double *Cval, int bshift, int tshift, int *err, hipStream_t s)
+{
+    constexpr int BS = G < BLOCK ? BLOCK : G;
+    constexpr int RPB = BS / G;
+    const int slice = (max_need + 15) & ~15;
+    int64_t blocks = ((int64_t)count + RPB - 1) / RPB;
+    if (blocks > (1 << 20))
+        blocks = 1 << 20;
+    const int lg = __builtin_ctz(G);
+    hipLaunchKernelGGL((k_numeric_lds<G>), dim3((unsigned)blocks), dim3(BS), (size_t)slice * RPB, s, rows, count_dev,
+                       slice, op, crp, Ccol, Cval, bshift > lg ? lg : bshift, tshift > lg ? lg : tshift, err);
+}
+
+hipError_t launch_numeric(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
+                          const long long *crp, int *Ccol, double *Cval, int bshift, int tshift, u64 *gws,
+                          long long gws_stride, long long gws_tiles, int gws_blocks, int *err, hipStream_t s)
+{
+    if (count <= 0)
+        return hipSuccess;
+    switch (bin) {
+    case 0: numeric_g<8>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 1: numeric_g<16>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 2: numeric_g<32>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 3: numeric_g<64>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 4: numeric_g<128>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 5: numeric_g<256>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 6: numeric_g<512>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 7: numeric_g<1024>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case BIN_GLOBAL: {
+        const int blocks = count < gws_blocks ? count : gws_blocks;
+        hipLaunchKernelGGL(k_numeric_global, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, crp, Ccol, Cval,
+                           bshift > 10 ? 10 : bshift, tshift > 10 ? 10 : tshift, gws, gws_stride, gws_tiles, err);
+        break;
+    }
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t configure_kernels(int lds_max)
+{
+    hipError_t e = hipSuccess;
+#define CFG(K)                                                                                                         \
+    {                                                                                                                  \
+        hipError_t r = hipFuncSetAttribute((const void *)(K), hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);    \
+        if (r != hipSuccess)                                                                                           \
+            e = r;                                                                                                     \
+    }
+    CFG(k_symbolic_lds<8>) CFG(k_symbolic_lds<16>) CFG(k_symbolic_lds<32>) CFG(k_symbolic_lds<64>)
+    CFG(k_symbolic_lds<128>) CFG(k_symbolic_lds<256>) CFG(k_symbolic_lds<512>) CFG(k_symbolic_lds<1024>)
+    CFG(k_numeric_lds<8>) CFG(k_numeric_lds<16>) CFG(k_numeric_lds<32>) CFG(k_numeric_lds<64>)
+    CFG(k_numeric_lds<128>) CFG(k_numeric_lds<256>) CFG(k_numeric_lds<512>) CFG(k_numeric_lds<1024>)
+#undef CFG
+    return e;
+}
+
+} // namespace hsmu

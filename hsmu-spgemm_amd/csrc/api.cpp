@@ -225,11 +225,11 @@ int hsmu_ctx_create(int device, hsmu_ctx **out)
         for (int d : configured)
             done |= d == device;
         if (!done) {
-            hipError_t e = configure_kernels(c->lds_max);
-            if (e != hipSuccess) {
-                hsmu_ctx_destroy(c);
-                return fail(HSMU_ERR_HIP, "configure_kernels: %s", hipGetErrorString(e));
-            }
+            // Best effort: HIP on AMD accepts dynamic LDS up to the device limit without
+            // the attribute; a refusal here is not an error, a launch that needs more
+            // LDS than allowed reports its own.
+            if (configure_kernels(c->lds_max - LDS_STATIC_RESERVE) != hipSuccess)
+                (void)hipGetLastError();
             configured.push_back(device);
         }
     }
@@ -589,7 +589,10 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     // lanes per A entry: walk one B row (numeric) / one B tile row (tile OR)
     const long long avg_b = B->nnz / (K > 0 ? K : 1);
     const long long avg_t = (long long)h1.ntiles_b / (K > 0 ? K : 1);
-    int blanes = opts.b_lanes > 0 ? opts.b_lanes : pow2_floor(avg_b > 0 ? avg_b : 1);
+    // about four products per lane per B row: shorter sub-groups keep more A entries in
+    // flight per wave and idle fewer lanes on ragged B rows (measured on the banded
+    // workload: 16 lanes for 64-entry rows beats 64 lanes by 1.2x in the numeric phase)
+    int blanes = opts.b_lanes > 0 ? opts.b_lanes : pow2_floor(avg_b / 4 > 0 ? avg_b / 4 : 1);
     int tlanes = opts.t_lanes > 0 ? opts.t_lanes : pow2_floor(avg_t > 0 ? avg_t : 1);
     if (blanes > 64)
         blanes = 64;
@@ -598,7 +601,9 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     const int bshift = ilog2(blanes), tshift = ilog2(tlanes);
 
     // global workspace for rows that do not fit LDS
-    const int max_span = h1.max_span > 0 ? h1.max_span : 1;
+    int max_span = 1;
+    for (int b = 0; b < NBINS; ++b)
+        max_span = h1.sym.max_span[b] > max_span ? h1.sym.max_span[b] : max_span;
     const long long gws_tiles = max_span;
     const long long gws_stride = (gws_tiles + (gws_tiles + 1) / 2 + 31) & ~31LL; // u64 units, 256 B aligned
     int gws_blocks = c->num_cus * 2;
@@ -654,7 +659,7 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
         }
     } crp_guard{c, crp};
     HIP_TRY(launch_scan_i32_i64(rnnz, M, crp, scan_ws, &dst->nnz_c, s));
-    bp.per_lane = opts.products_per_lane > 0 ? opts.products_per_lane : 8;
+    bp.per_lane = opts.products_per_lane > 0 ? opts.products_per_lane : 32;
     HIP_TRY(launch_bin(1, M, (const int *)A->rowptr, rprod, rtp, rmin, rmax, rnnz, bp, binrows, dst, s));
     HIP_TRY(hipEventRecord(c->ev[EV_SYMBOLIC], s));
     HIP_TRY(hipMemcpyAsync(c->host_stats, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
@@ -740,11 +745,16 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
         S.ms_total = elapsed(c->ev[EV_START], c->ev[EV_NUMERIC]);
         for (int b = 0; b < NBINS; ++b)
             S.ms_numeric_bin[b] = (timing && h2.num.count[b] > 0) ? elapsed(c->evb[b][0], c->evb[b][1]) : 0.0;
-        S.products = (int64_t)h2.products;
-        S.tile_products = (int64_t)h2.tile_products;
+        S.products = 0;
+        S.tile_products = 0;
+        S.rows_nonempty = 0;
+        for (int b = 0; b < NBINS; ++b) {
+            S.products += (int64_t)h2.num.work[b];
+            S.tile_products += (int64_t)h2.sym.work[b];
+            S.rows_nonempty += h2.sym.count[b];
+        }
         S.nnz_c = nnzc;
         S.tiles_b = h2.ntiles_b;
-        S.rows_nonempty = (int64_t)h2.rows_nonempty;
         for (int b = 0; b < NBINS; ++b) {
             S.sym_bin_rows[b] = h2.sym.count[b];
             S.num_bin_rows[b] = h2.num.count[b];

@@ -1,0 +1,126 @@
+// device_common.h -- wave64 / workgroup helpers shared by the HIP kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hsmu {
+
+typedef unsigned long long u64;
+
+#define DEVI __device__ __forceinline__
+
+// ---- wave64 helpers ---------------------------------------------------------------
+
+// number of set bits of m below this lane (v_mbcnt_lo/hi)
+DEVI int mbcnt(u64 m)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+DEVI u64 below(int b) { return (1ull << b) - 1ull; }
+
+// Map the hardware block index so that each XCD (blocks are dealt round-robin over the
+// 8 XCDs) works on a contiguous range of logical blocks: neighbouring rows, which share
+// rows of B, then share one XCD's L2.
+DEVI int xcd_block(int b, int n)
+{
+    const int x = b & 7, q = b >> 3, per = n >> 3, rem = n & 7;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
+// Sum over a group of G consecutive lanes.  G <= 64: shuffles; G > 64: through LDS
+// (every thread of the block must call it).
+template <int G, typename T> DEVI T group_sum(T v, T *red)
+{
+    if constexpr (G <= 64) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1)
+            v += __shfl_xor(v, o, G);
+        return v;
+    } else {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+            v += __shfl_xor(v, o, 64);
+        const int w = threadIdx.x >> 6;
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0)
+            red[w] = v;
+        __syncthreads();
+        const int w0 = (threadIdx.x / G) * (G / 64);
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < G / 64; ++k)
+            s += red[w0 + k];
+        return s;
+    }
+}
+
+template <int G, typename T> DEVI T group_min(T v)
+{
+    static_assert(G <= 64, "group_min is a wave operation");
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        T y = __shfl_xor(v, o, G);
+        v = y < v ? y : v;
+    }
+    return v;
+}
+
+template <int G, typename T> DEVI T group_max(T v)
+{
+    static_assert(G <= 64, "group_max is a wave operation");
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        T y = __shfl_xor(v, o, G);
+        v = y > v ? y : v;
+    }
+    return v;
+}
+
+// Exclusive scan over a group of G lanes; total = group sum.  G > 64 goes through LDS
+// (every thread of the block must call it).
+template <int G, typename T> DEVI T group_excl_scan(T v, T *red, T &total)
+{
+    constexpr int W = G < 64 ? G : 64;
+    const int l = threadIdx.x & (W - 1);
+    T x = v;
+#pragma unroll
+    for (int o = 1; o < W; o <<= 1) {
+        T y = __shfl_up(x, o, W);
+        if (l >= o)
+            x += y;
+    }
+    if constexpr (G <= 64) {
+        total = __shfl(x, W - 1, W);
+        return x - v;
+    } else {
+        const int w = threadIdx.x >> 6;
+        __syncthreads();
+        if (l == 63)
+            red[w] = x;
+        __syncthreads();
+        const int w0 = (threadIdx.x / G) * (G / 64);
+        T before = 0, tot = 0;
+#pragma unroll
+        for (int k = 0; k < G / 64; ++k) {
+            const T r = red[w0 + k];
+            if (w0 + k < w)
+                before += r;
+            tot += r;
+        }
+        total = tot;
+        return before + x - v;
+    }
+}
+
+// Workgroup barrier that also orders global memory for the whole workgroup at device
+// scope: prior stores are complete in L2 and this CU's L1 is invalidated afterwards,
+// so plain loads see values other waves wrote with stores or L2 atomics.
+DEVI void block_sync_global()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+} // namespace hsmu

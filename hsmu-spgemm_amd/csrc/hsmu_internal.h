@@ -56,20 +56,59 @@ struct DevStats {
     BinStats num;
 };
 
-// LDS bytes a row needs in each phase; S = tiles spanned, n = nnz of the row of C.
-// Symbolic: one 64-bit mask per tile.  Numeric: mask + 32-bit offset per tile and one
-// FP64 (or two-float) accumulator per nonzero.
 #if defined(__HIPCC__)
 #define HSMU_HD __host__ __device__
 #else
 #define HSMU_HD
 #endif
-// (rounded up to 16 bytes: per-row slices of a workgroup stay 16-byte aligned)
-HSMU_HD inline long long sym_need(long long S) { return (8 * S + 15) & ~15LL; }
-HSMU_HD inline long long num_need(long long S, long long n)
+
+// LDS slice of one row in the LDS kernels (byte offsets), for a row whose C row spans
+// S tiles, has n = nnz(C_i) entries and nA entries in A:
+//   dm   u64[S]     tile masks of the row of C over its span [rmin, rmax]
+//   doff u32[S]     numeric: slot of each tile's first column
+//   vals f64[n]     numeric: accumulators, exactly nnz(C_i) of them
+//   av   f64[nA]    numeric: a_ij of the row's entries
+//   tlen i32[nA]    tiles(B_j) for each entry j of the row
+//   t0   i32[nA]    first tile of B_j
+//   blen i32[nA]    numeric: nnz(B_j)
+//   b0   i32[nA]    numeric: first stored element of B_j
+// The size is rounded up to 16 bytes so per-row slices of a workgroup stay aligned.
+struct RowLayout {
+    long long dm, doff, vals, av, tlen, t0, blen, b0, bytes;
+};
+
+HSMU_HD inline RowLayout row_layout(int numeric, long long S, long long n, long long nA)
 {
-    return ((((12 * S + 7) & ~7LL) + 8 * n) + 15) & ~15LL;
+    RowLayout L;
+    long long o = 0;
+    L.dm = o;
+    o += 8 * S;
+    L.doff = o;
+    if (numeric)
+        o += 4 * S;
+    o = (o + 7) & ~7LL;
+    L.vals = o;
+    if (numeric)
+        o += 8 * n;
+    L.av = o;
+    if (numeric)
+        o += 8 * nA;
+    L.tlen = o;
+    o += 4 * nA;
+    L.t0 = o;
+    o += 4 * nA;
+    L.blen = o;
+    if (numeric)
+        o += 4 * nA;
+    L.b0 = o;
+    if (numeric)
+        o += 4 * nA;
+    L.bytes = (o + 15) & ~15LL;
+    return L;
 }
+
+HSMU_HD inline long long sym_need(long long S, long long nA) { return row_layout(0, S, 0, nA).bytes; }
+HSMU_HD inline long long num_need(long long S, long long n, long long nA) { return row_layout(1, S, n, nA).bytes; }
 
 // LDS kept back from the dynamic allocation for the kernels' static __shared__ arrays
 constexpr int LDS_STATIC_RESERVE = 256;

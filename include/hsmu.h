@@ -110,11 +110,11 @@ typedef struct hsmu_opts {
                                  global-memory path; 0 = device maximum */
     int32_t lds_block_budget; /* LDS bytes per workgroup for bins that pack several rows
                                  into one workgroup; 0 = default (64 KiB) */
-    int32_t products_per_lane;/* numeric work per lane that sets a row's lane count; 0 = 8 */
+    int32_t products_per_lane;/* numeric work per lane that sets a row's lane count; 0 = 32 */
     int32_t tiles_per_lane;   /* symbolic work per lane that sets a row's lane count; 0 = 4 */
     int32_t timing;           /* 1: record per-stage hipEvents into hsmu_stats */
     int32_t b_lanes;          /* lanes that walk one row of B in the numeric phase (power of
-                                 two, 1..64); 0 = from the mean row length of B */
+                                 two, 1..64); 0 = a quarter of the mean row length of B */
     int32_t t_lanes;          /* lanes that walk one row of B's tiles in the tile OR (power
                                  of two, 1..64); 0 = from the mean tiles per row of B */
     int32_t reserved[5];
