@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--opts", default="", help="hsmu_opts overrides, e.g. products_per_lane=16,b_lanes=32")
+    ap.add_argument("--copy-mib", type=int, default=1024,
+                    help="size of the device-to-device copy probe (achievable HBM bandwidth; 0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -55,6 +57,12 @@ def main():
     log(f"generated banded A: {A.M} rows, nnz {A.nnz} in {time.time() - t0:.1f}s")
     ctx = hsmu.Context(0)
     dA = ctx.matrix(A)
+    copy_gbs = None
+    if args.copy_mib > 0:
+        nbytes = args.copy_mib << 20
+        ms = ctx.copy_kernel_ms(nbytes)
+        copy_gbs = 2.0 * nbytes / (ms * 1e-3) / 1e9  # read + write
+        log(f"copy probe: {args.copy_mib} MiB in {ms:.3f} ms = {copy_gbs:.0f} GB/s (read+write)")
     overrides = dict(kv.split("=") for kv in args.opts.split(",") if kv)
     opts = hsmu.default_opts(**{k: int(v) for k, v in overrides.items()})
 
@@ -134,7 +142,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": f"k_numeric_lds bin {dom}" if dom < 8 else "k_numeric_global",
-                     "kernel_ms": round(dom_ms, 4), "alg_bytes_per_launch": dom_bytes},
+                     "kernel_ms": round(dom_ms, 4), "alg_bytes_per_launch": dom_bytes,
+                     "copy_kernel_GBps": round(copy_gbs, 1) if copy_gbs else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -35,9 +35,11 @@ DEVI void stage_row(const Operands &op, int a0, int nA, int *t0, int *tlen, int 
     const int l = threadIdx.x & (G - 1);
     for (int k = l; k < nA; k += G) {
         const int j = op.Acol[a0 + k];
-        const int tt = op.tptr[j];
-        t0[k] = tt;
-        tlen[k] = op.tptr[j + 1] - tt;
+        if (t0) {
+            const int tt = op.tptr[j];
+            t0[k] = tt;
+            tlen[k] = op.tptr[j + 1] - tt;
+        }
         if constexpr (NUMERIC) {
             const int bb = op.Brp[j];
             b0[k] = bb;
@@ -83,7 +85,7 @@ DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, u
 // Add every product a_ij * b_jc of the row into its slot: sub-groups of 2^bshift lanes
 // take one entry each and walk B_j, UNROLL elements per lane per iteration with all
 // loads issued before the ds_add_f64s.
-template <int G>
+template <int G, bool DENSE>
 DEVI void add_products(const Operands &op, const int *b0, const int *blen, const double *av, int nA, const u64 *dm,
                        const unsigned *doff, double *vals, int mn, int bshift)
 {
@@ -108,8 +110,13 @@ DEVI void add_products(const Operands &op, const int *b0, const int *blen, const
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
                 if (c[u] >= 0) {
-                    const int t = (c[u] >> 6) - mn;
-                    const unsigned slot = doff[t] + (unsigned)__popcll(dm[t] & below(c[u] & 63));
+                    unsigned slot;
+                    if constexpr (DENSE) {
+                        slot = (unsigned)(c[u] - (mn << 6));
+                    } else {
+                        const int t = (c[u] >> 6) - mn;
+                        slot = doff[t] + (unsigned)__popcll(dm[t] & below(c[u] & 63));
+                    }
                     atomicAdd(&vals[slot], a * v[u]);
                 }
             }
@@ -129,7 +136,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ int redi[16];
+    __shared__ unsigned redu[16];
     const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
     unsigned char *slice = lds + (size_t)g * slice_bytes;
     const int nrows = *count_p;
@@ -153,12 +160,27 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
         __syncthreads();
         or_tiles<G>(op, t0, tlen, nA, dm, mn, tshift);
         __syncthreads();
-        int cnt = 0;
-        for (int t = l; t < S; t += G)
-            cnt += __popcll(dm[t]);
-        cnt = group_sum<G>(cnt, redi);
+        // nnz(C_i) = sum of popcounts; when the directory is kept, the exclusive prefix
+        // (slot of each tile's first column) goes out with the masks for the numeric phase
+        const int chunk = (S + G - 1) / G;
+        const int ta = l * chunk;
+        const int tb = ta + chunk < S ? ta + chunk : S;
+        unsigned loc = 0;
+        for (int t = ta; t < tb; ++t)
+            loc += (unsigned)__popcll(dm[t]);
+        unsigned tot;
+        unsigned pre = group_excl_scan<G>(loc, redu, tot);
+        const long long rd = (row >= 0 && op.dir_mask) ? op.rdir[row] : -1;
+        if (rd >= 0) {
+            for (int t = ta; t < tb; ++t) {
+                const u64 m = dm[t];
+                op.dir_mask[rd + t] = m;
+                op.dir_off[rd + t] = pre;
+                pre += (unsigned)__popcll(m);
+            }
+        }
         if (row >= 0 && l == 0)
-            rnnz[row] = cnt;
+            rnnz[row] = (int)tot;
         __syncthreads();
     }
 }
@@ -243,15 +265,29 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
         int *t0 = (int *)(slice + L.t0);
         int *blen = (int *)(slice + L.blen);
         int *b0 = (int *)(slice + L.b0);
-        for (int t = l; t < S; t += G)
-            dm[t] = 0;
-        for (int q = l; q < n; q += G)
+        const bool dense = use_dense(S, n);
+        const int nv = dense ? 64 * S : n;
+        const long long rd = (row >= 0 && op.dir_mask) ? op.rdir[row] : -1;
+        for (int q = l; q < nv; q += G)
             vals[q] = 0.0;
-        stage_row<G, true>(op, a0, nA, t0, tlen, b0, blen, av);
+        if (rd >= 0) {
+            // directory kept by the symbolic phase: load it, stage only B's ranges
+            for (int t = l; t < S; t += G) {
+                dm[t] = op.dir_mask[rd + t];
+                doff[t] = op.dir_off[rd + t];
+            }
+            stage_row<G, true>(op, a0, nA, nullptr, nullptr, b0, blen, av);
+        } else {
+            for (int t = l; t < S; t += G)
+                dm[t] = 0;
+            stage_row<G, true>(op, a0, nA, t0, tlen, b0, blen, av);
+        }
         __syncthreads();
-        or_tiles<G>(op, t0, tlen, nA, dm, mn, tshift);
+        if (rd < 0)
+            or_tiles<G>(op, t0, tlen, nA, dm, mn, tshift);
         __syncthreads();
-        // slot offsets: each lane scans a contiguous chunk of tiles
+        // slot offsets (rebuilt directories) and the nnz check (all rows): each lane
+        // scans a contiguous chunk of tiles
         const int chunk = (S + G - 1) / G;
         const int ta = l * chunk;
         const int tb = ta + chunk < S ? ta + chunk : S;
@@ -260,16 +296,22 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
             loc += (unsigned)__popcll(dm[t]);
         unsigned tot;
         unsigned pre = group_excl_scan<G>(loc, redu, tot);
-        for (int t = ta; t < tb; ++t) {
-            doff[t] = pre;
-            pre += (unsigned)__popcll(dm[t]);
+        if (rd < 0) {
+            for (int t = ta; t < tb; ++t) {
+                doff[t] = pre;
+                pre += (unsigned)__popcll(dm[t]);
+            }
         }
         const bool ok = (int)tot == n;
         if (row >= 0 && !ok && l == 0)
             atomicOr(err, ERR_INTERNAL);
         __syncthreads();
-        if (row >= 0 && ok)
-            add_products<G>(op, b0, blen, av, nA, dm, doff, vals, mn, bshift);
+        if (row >= 0 && ok) {
+            if (dense)
+                add_products<G, true>(op, b0, blen, av, nA, dm, doff, vals, mn, bshift);
+            else
+                add_products<G, false>(op, b0, blen, av, nA, dm, doff, vals, mn, bshift);
+        }
         __syncthreads();
         if (row >= 0 && ok) {
             if constexpr (G >= 64) {
@@ -280,7 +322,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
                     if ((m >> lane) & 1ull) {
                         const unsigned pos = doff[t] + (unsigned)__popcll(m & below(lane));
                         Ccol[cb + pos] = ((mn + t) << 6) + lane;
-                        Cval[cb + pos] = vals[pos];
+                        Cval[cb + pos] = vals[dense ? (t << 6) + lane : (int)pos];
                     }
                 }
             } else {
@@ -291,12 +333,15 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
                     while (m) {
                         const int b = __ffsll((unsigned long long)m) - 1;
                         Ccol[cb + pos] = c0 + b;
+                        if (dense)
+                            Cval[cb + pos] = vals[(t << 6) + b];
                         ++pos;
                         m &= m - 1;
                     }
                 }
-                for (int q = l; q < n; q += G)
-                    Cval[cb + q] = vals[q];
+                if (!dense)
+                    for (int q = l; q < n; q += G)
+                        Cval[cb + q] = vals[q];
             }
         }
         __syncthreads();

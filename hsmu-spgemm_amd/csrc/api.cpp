@@ -552,7 +552,9 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     int *rmax = ws.get<int>((size_t)M);
     int *rnnz = ws.get<int>((size_t)M);
     int *binrows = ws.get<int>((size_t)NBINS * (size_t)M);
-    if (!dst || !tcnt || !tptr || !tcol || !tmask || !scan_ws || !rprod || !rtp || !rmin || !rmax || !rnnz || !binrows)
+    long long *rdir = ws.get<long long>((size_t)M);
+    if (!dst || !tcnt || !tptr || !tcol || !tmask || !scan_ws || !rprod || !rtp || !rmin || !rmax || !rnnz || !binrows ||
+        !rdir)
         return fail(HSMU_ERR_NOMEM, "workspace allocation");
 
     // ---- 1. B -> tiles ----
@@ -577,7 +579,7 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     bp.per_lane = opts.tiles_per_lane > 0 ? opts.tiles_per_lane : 4;
     bp.force_global = opts.force_global;
     bp.pad = 0;
-    HIP_TRY(launch_bin(0, M, (const int *)A->rowptr, rprod, rtp, rmin, rmax, rnnz, bp, binrows, dst, s));
+    HIP_TRY(launch_bin(0, M, (const int *)A->rowptr, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, s));
     HIP_TRY(hipEventRecord(c->ev[EV_ANALYSIS], s));
     HIP_TRY(hipMemcpyAsync(c->host_stats, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -631,6 +633,22 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     op.tmask = tmask;
     op.rmin = rmin;
     op.rmax = rmax;
+    // Row directories (tile masks + slot offsets) of LDS-bin rows are kept from the
+    // symbolic to the numeric phase when they fit the budget (12 bytes per spanned tile:
+    // 15 MB on the banded workload); otherwise the numeric kernel rebuilds them.
+    op.rdir = rdir;
+    op.dir_mask = nullptr;
+    op.dir_off = nullptr;
+    const unsigned long long dir_total = h1.dir_total;
+    const unsigned long long dir_budget = 1ull << 31;
+    if (!opts.rebuild_dirs && dir_total > 0 && dir_total * 12ull <= dir_budget) {
+        op.dir_mask = ws.get<unsigned long long>((size_t)dir_total);
+        op.dir_off = ws.get<unsigned>((size_t)dir_total);
+        if (!op.dir_mask || !op.dir_off) {
+            op.dir_mask = nullptr;
+            op.dir_off = nullptr;
+        }
+    }
 
     // ---- 3. symbolic ----
     for (int b = 0; b < NBINS; ++b) {
@@ -660,7 +678,7 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     } crp_guard{c, crp};
     HIP_TRY(launch_scan_i32_i64(rnnz, M, crp, scan_ws, &dst->nnz_c, s));
     bp.per_lane = opts.products_per_lane > 0 ? opts.products_per_lane : 32;
-    HIP_TRY(launch_bin(1, M, (const int *)A->rowptr, rprod, rtp, rmin, rmax, rnnz, bp, binrows, dst, s));
+    HIP_TRY(launch_bin(1, M, (const int *)A->rowptr, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, s));
     HIP_TRY(hipEventRecord(c->ev[EV_SYMBOLIC], s));
     HIP_TRY(hipMemcpyAsync(c->host_stats, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

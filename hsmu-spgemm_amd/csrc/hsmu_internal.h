@@ -44,14 +44,10 @@ struct BinStats {
 // Device-side counters of one call, copied to pinned host memory at the two
 // synchronisation points.
 struct DevStats {
-    unsigned long long products;
-    unsigned long long tile_products;
-    unsigned long long rows_nonempty;
+    unsigned long long dir_total; // directory entries reserved for rows in LDS bins
     long long nnz_c;
     int err;
     int ntiles_b;
-    int max_span;
-    int pad;
     BinStats sym;
     BinStats num;
 };
@@ -62,11 +58,18 @@ struct DevStats {
 #define HSMU_HD
 #endif
 
+// Accumulator form of a row in the numeric LDS kernel.  Compact (the HSMU form): exactly
+// nnz(C_i) slots, column c in slot doff[t] + popcount(dm[t] & below(c & 63)).  Dense:
+// one slot per column of the row's span, column c in slot c - 64*rmin, used when the
+// span is at least two-thirds full (then the dense array costs at most 1.5x the compact
+// one and saves the two directory reads per product) and at most 128 tiles wide.
+HSMU_HD inline bool use_dense(long long S, long long n) { return S <= 128 && 2 * 64 * S <= 3 * n; }
+
 // LDS slice of one row in the LDS kernels (byte offsets), for a row whose C row spans
 // S tiles, has n = nnz(C_i) entries and nA entries in A:
 //   dm   u64[S]     tile masks of the row of C over its span [rmin, rmax]
 //   doff u32[S]     numeric: slot of each tile's first column
-//   vals f64[n]     numeric: accumulators, exactly nnz(C_i) of them
+//   vals f64[n]     numeric: accumulators (64*S of them in the dense form)
 //   av   f64[nA]    numeric: a_ij of the row's entries
 //   tlen i32[nA]    tiles(B_j) for each entry j of the row
 //   t0   i32[nA]    first tile of B_j
@@ -89,7 +92,7 @@ HSMU_HD inline RowLayout row_layout(int numeric, long long S, long long n, long 
     o = (o + 7) & ~7LL;
     L.vals = o;
     if (numeric)
-        o += 8 * n;
+        o += 8 * (use_dense(S, n) ? 64 * S : n);
     L.av = o;
     if (numeric)
         o += 8 * nA;
@@ -130,10 +133,12 @@ hipError_t launch_row_info(int64_t M, int64_t K, int64_t nnzA, const int *Arp, c
                            int *rtp, int *rmin, int *rmax, int *rnnz, DevStats *st, int avg_a,
                            hipStream_t s);
 
-// Bin rows for the symbolic (numeric = 0) or numeric (numeric = 1) phase.
+// Bin rows for the symbolic (numeric = 0) or numeric (numeric = 1) phase.  The symbolic
+// pass also reserves each LDS-bin row's directory range: rdir[i] = offset (span entries),
+// -1 for rows not in an LDS bin; st->dir_total = entries reserved.
 hipError_t launch_bin(int numeric, int64_t M, const int *Arp, const long long *rprod,
                       const int *rtp, const int *rmin, const int *rmax, const int *rnnz,
-                      BinParams bp, int *binrows, DevStats *st, hipStream_t s);
+                      BinParams bp, int *binrows, long long *rdir, DevStats *st, hipStream_t s);
 
 struct Operands {
     const int *Arp, *Acol;
@@ -143,6 +148,12 @@ struct Operands {
     const int *tptr, *tcol;
     const unsigned long long *tmask;
     const int *rmin, *rmax;
+    // Row directories kept between the phases (nullptr when not kept): the symbolic LDS
+    // kernel stores row i's tile masks and slot offsets at dir_mask/dir_off + rdir[i]
+    // (rdir < 0: not stored), the numeric kernel loads them instead of rebuilding them.
+    const long long *rdir;
+    unsigned long long *dir_mask;
+    unsigned *dir_off;
 };
 
 // Global-memory workspace (bin BIN_GLOBAL): gws_blocks regions of gws_stride u64 each;

@@ -429,10 +429,11 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
                                               const long long *__restrict__ rprod, const int *__restrict__ rtp,
                                               const int *__restrict__ rmin, const int *__restrict__ rmax,
                                               const int *__restrict__ rnnz, BinParams bp, int *__restrict__ binrows,
-                                              DevStats *st)
+                                              long long *__restrict__ rdir, DevStats *st)
 {
     __shared__ int s_cnt[NBINS], s_need[NBINS], s_span[NBINS], s_base[NBINS];
     __shared__ unsigned long long s_work[NBINS], s_nc[NBINS], s_na[NBINS];
+    __shared__ unsigned long long s_dir, s_dir_base;
     if (threadIdx.x < NBINS) {
         s_cnt[threadIdx.x] = 0;
         s_need[threadIdx.x] = 0;
@@ -441,9 +442,12 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
         s_nc[threadIdx.x] = 0;
         s_na[threadIdx.x] = 0;
     }
+    if (threadIdx.x == 0)
+        s_dir = 0;
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
     int b = -1, pos = 0;
+    long long dloc = -1;
     if (i < M && rprod[i] > 0) {
         const int S = rmax[i] - rmin[i] + 1;
         const int nA = Arp[i + 1] - Arp[i];
@@ -464,6 +468,8 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
         atomicAdd(&s_work[b], (unsigned long long)work);
         atomicAdd(&s_nc[b], (unsigned long long)nc);
         atomicAdd(&s_na[b], (unsigned long long)nA);
+        if (!numeric && b < BIN_GLOBAL)
+            dloc = (long long)atomicAdd(&s_dir, (unsigned long long)S);
     }
     __syncthreads();
     BinStats *bs = numeric ? &st->num : &st->sym;
@@ -478,19 +484,24 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
             atomicAdd(&bs->nnz_a[threadIdx.x], s_na[threadIdx.x]);
         }
     }
+    if (!numeric && threadIdx.x == NBINS)
+        s_dir_base = s_dir ? atomicAdd(&st->dir_total, s_dir) : 0;
     __syncthreads();
     if (b >= 0)
         binrows[(int64_t)b * M + s_base[b] + pos] = (int)i;
+    if (!numeric && i < M)
+        rdir[i] = dloc >= 0 ? (long long)s_dir_base + dloc : -1;
 }
 
 hipError_t launch_bin(int numeric, int64_t M, const int *Arp, const long long *rprod, const int *rtp, const int *rmin,
-                      const int *rmax, const int *rnnz, BinParams bp, int *binrows, DevStats *st, hipStream_t s)
+                      const int *rmax, const int *rnnz, BinParams bp, int *binrows, long long *rdir, DevStats *st,
+                      hipStream_t s)
 {
     const int64_t blocks = (M + 1023) / 1024;
     if (blocks == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_bin, dim3((unsigned)blocks), dim3(1024), 0, s, numeric, M, Arp, rprod, rtp, rmin, rmax, rnnz,
-                       bp, binrows, st);
+                       bp, binrows, rdir, st);
     return hipGetLastError();
 }
 

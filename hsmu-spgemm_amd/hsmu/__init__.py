@@ -66,7 +66,8 @@ class Opts(C.Structure):
     _fields_ = [("precision", C.c_int32), ("rowptr64", C.c_int32), ("plain_alloc", C.c_int32),
                 ("force_global", C.c_int32), ("lds_row_max", C.c_int32), ("lds_block_budget", C.c_int32),
                 ("products_per_lane", C.c_int32), ("tiles_per_lane", C.c_int32), ("timing", C.c_int32),
-                ("b_lanes", C.c_int32), ("t_lanes", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("b_lanes", C.c_int32), ("t_lanes", C.c_int32), ("rebuild_dirs", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):

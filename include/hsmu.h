@@ -117,7 +117,9 @@ typedef struct hsmu_opts {
                                  two, 1..64); 0 = a quarter of the mean row length of B */
     int32_t t_lanes;          /* lanes that walk one row of B's tiles in the tile OR (power
                                  of two, 1..64); 0 = from the mean tiles per row of B */
-    int32_t reserved[5];
+    int32_t rebuild_dirs;     /* 1: the numeric phase rebuilds each row's tile directory
+                                 instead of reusing the one the symbolic phase kept */
+    int32_t reserved[4];
 } hsmu_opts;
 
 #define HSMU_NBINS 10 /* 8 LDS bins (8..1024 lanes per row), global fallback, wide rows */
