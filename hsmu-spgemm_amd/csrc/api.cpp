@@ -448,8 +448,8 @@ extern "C" int hsmu_btiles(hsmu_ctx *c, const hsmu_csr *B, hsmu_tiles *T)
     memset(T, 0, sizeof(*T));
     const int64_t K = B->M;
     Scratch ws(c->pool);
-    int *tcnt = ws.get<int>((size_t)K + 1);
-    void *scan_ws = ws.get<char>(scan_ws_bytes(K));
+    int *tcnt = (int *)ws.get<char>(btiles_ws_bytes(B->nnz));
+    void *scan_ws = ws.get<char>(scan_ws_bytes(B->nnz + 1));
     DevStats *dst = ws.get<DevStats>(1);
     int *tptr = (int *)c->pool.alloc(((size_t)K + 1) * sizeof(int));
     int *tcol = (int *)c->pool.alloc((size_t)(B->nnz > 0 ? B->nnz : 1) * sizeof(int));
@@ -540,11 +540,12 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     const bool timing = opts.timing != 0;
     Scratch ws(c->pool);
     DevStats *dst = ws.get<DevStats>(1);
-    int *tcnt = ws.get<int>((size_t)K + 1);
+    int *tcnt = (int *)ws.get<char>(btiles_ws_bytes(B->nnz));
     int *tptr = ws.get<int>((size_t)K + 1);
     int *tcol = ws.get<int>((size_t)B->nnz);
     unsigned long long *tmask = ws.get<unsigned long long>((size_t)B->nnz);
-    const int64_t nmax = M > K ? M : K;
+    int64_t nmax = M > K ? M : K;
+    nmax = nmax > B->nnz ? nmax : B->nnz;
     void *scan_ws = ws.get<char>(scan_ws_bytes(nmax + 1));
     long long *rprod = ws.get<long long>((size_t)M);
     int *rtp = ws.get<int>((size_t)M);
@@ -566,8 +567,9 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
 
     // ---- 2. row analysis + symbolic binning ----
     const int avg_a = (int)(A->nnz / (M > 0 ? M : 1));
+    // the spare bin region (bin 9) holds the list of long rows until binning runs
     HIP_TRY(launch_row_info(M, K, A->nnz, (const int *)A->rowptr, A->col, (const int *)B->rowptr, tptr, tcol, rprod,
-                            rtp, rmin, rmax, rnnz, dst, avg_a, s));
+                            rtp, rmin, rmax, rnnz, binrows + (size_t)(NBINS - 1) * M, dst, avg_a, c->num_cus, s));
     BinParams bp;
     const int lds_dyn = c->lds_max - LDS_STATIC_RESERVE;
     bp.lds_row_max = opts.lds_row_max > 0 ? opts.lds_row_max : lds_dyn;

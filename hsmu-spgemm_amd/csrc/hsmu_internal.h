@@ -48,6 +48,8 @@ struct DevStats {
     long long nnz_c;
     int err;
     int ntiles_b;
+    int nlong; // rows of A set aside for the one-workgroup-per-row analysis
+    int pad;
     BinStats sym;
     BinStats num;
 };
@@ -123,15 +125,18 @@ hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 // B -> tiles: per-row tile counts, scan to tptr, fill tcol/tmask.
 // scan_ws must hold scan_ws_bytes(K) bytes.
 size_t scan_ws_bytes(int64_t n);
+// ws must hold btiles_ws_bytes(nnz) bytes, scan_ws scan_ws_bytes(nnz + 1).
+size_t btiles_ws_bytes(int64_t nnz);
 hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const int *col,
-                         int *tcnt, int *tptr, int *tcol, unsigned long long *tmask, void *scan_ws,
+                         int *ws, int *tptr, int *tcol, unsigned long long *tmask, void *scan_ws,
                          DevStats *st, int num_cus, hipStream_t s);
 
-// Per row of A: products, tile-products, tile span; totals into st.
+// Per row of A: products, tile-products, tile span.  longrows (M ints) is scratch for the
+// list of long rows analysed one workgroup per row.
 hipError_t launch_row_info(int64_t M, int64_t K, int64_t nnzA, const int *Arp, const int *Acol,
                            const int *Brp, const int *tptr, const int *tcol, long long *rprod,
-                           int *rtp, int *rmin, int *rmax, int *rnnz, DevStats *st, int avg_a,
-                           hipStream_t s);
+                           int *rtp, int *rmin, int *rmax, int *rnnz, int *longrows, DevStats *st,
+                           int avg_a, int num_cus, hipStream_t s);
 
 // Bin rows for the symbolic (numeric = 0) or numeric (numeric = 1) phase.  The symbolic
 // pass also reserves each LDS-bin row's directory range: rdir[i] = offset (span entries),

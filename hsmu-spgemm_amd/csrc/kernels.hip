@@ -196,121 +196,110 @@ hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *s
 }
 
 // ---- B -> tiles ------------------------------------------------------------------------
+//
+// Element-parallel, so a row of any length costs the same per element (R-MAT's hub rows
+// made a row-per-group conversion wait on its longest row):
+//   1. k_bmark:  per row, validate the row pointers and flag the row's first element;
+//   2. k_bhead:  per element, validate the column and sortedness and flag tile heads
+//                (first element of a row, or tile column different from the left neighbour);
+//   3. exclusive scan of the flags: the tile index of every head;
+//   4. k_bfill:  per head, write its tile column and OR the bits of its run;
+//      k_btptr:  per row, tptr[j] = tile index at the row's first element.
+// Reference: compute_tile64_nums / compute_tile64_col / compute_tile64_mask_num,
+// src/compressed/compressed_mask64.cuh:61-342.
 
-// Tiles per row of B: a stored column starts a tile when its tile column differs from
-// its left neighbour's.  Also validates row pointers, column range and sortedness.
-template <int G>
-__global__ __launch_bounds__(256) void k_btile_count(int64_t K, int64_t N, int64_t nnz, const int *__restrict__ rp,
-                                                     const int *__restrict__ col, int *__restrict__ tcnt, DevStats *st)
+__global__ __launch_bounds__(256) void k_bmark(int64_t K, int64_t nnz, const int *__restrict__ rp,
+                                               int *__restrict__ flag, DevStats *st)
 {
-    constexpr int RPB = 256 / G;
-    const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
-    int bad = 0;
-    for (int64_t j = (int64_t)blockIdx.x * RPB + g; j < K; j += (int64_t)gridDim.x * RPB) {
-        const int s = rp[j], e = rp[j + 1];
-        int c = 0;
-        if (s < 0 || e < s || (int64_t)e > nnz) {
-            bad |= ERR_ROWPTR;
-        } else {
-            for (int k = s + l; k < e; k += G) {
-                const int cc = col[k];
-                if ((unsigned)cc >= (unsigned long long)N)
-                    bad |= ERR_BCOL;
-                if (k == s) {
-                    ++c;
-                } else {
-                    const int pc = col[k - 1];
-                    if (pc > cc)
-                        bad |= ERR_UNSORTED;
-                    if ((pc >> 6) != (cc >> 6))
-                        ++c;
-                }
-            }
-        }
-        c = group_sum<G>(c, (int *)nullptr);
-        if (l == 0)
-            tcnt[j] = c;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= K)
+        return;
+    const int s = rp[j], e = rp[j + 1];
+    if (s < 0 || e < s || (int64_t)e > nnz) {
+        atomicOr(&st->err, ERR_ROWPTR);
+        return;
     }
-    if (bad)
+    if (e > s)
+        flag[s] = 2;
+}
+
+__global__ __launch_bounds__(256) void k_bhead(int64_t nnz, int64_t N, const int *__restrict__ col, int *__restrict__ flag,
+                                               DevStats *st)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int bad = 0;
+    if (k < nnz) {
+        const int c = col[k];
+        if ((unsigned)c >= (unsigned long long)N)
+            bad |= ERR_BCOL;
+        const bool start = flag[k] == 2;
+        int head = 1;
+        if (!start && k > 0) {
+            const int pc = col[k - 1];
+            if (pc > c)
+                bad |= ERR_UNSORTED;
+            head = (pc >> 6) != (c >> 6);
+        }
+        flag[k] = head;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        bad |= __shfl_xor(bad, o, 64);
+    if (bad && (threadIdx.x & 63) == 0)
         atomicOr(&st->err, bad);
 }
 
-// Write tile columns and masks.  Heads (first element of each tile) get their tile
-// index from a ballot prefix count; each head ORs its tile's bits.
-template <int G>
-__global__ __launch_bounds__(256) void k_btile_fill(int64_t K, int64_t nnz, const int *__restrict__ rp,
-                                                    const int *__restrict__ col, const int *__restrict__ tptr,
-                                                    int *__restrict__ tcol, u64 *__restrict__ tmask)
+__global__ __launch_bounds__(256) void k_bfill(int64_t nnz, const int *__restrict__ col, const int *__restrict__ flag,
+                                               const int *__restrict__ tix, int *__restrict__ tcol, u64 *__restrict__ tmask)
 {
-    constexpr int RPB = 256 / G;
-    const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
-    const int lane = threadIdx.x & 63;
-    u64 gmask = ~0ull;
-    if constexpr (G < 64)
-        gmask = ((1ull << G) - 1ull) << (lane & ~(G - 1));
-    for (int64_t j = (int64_t)blockIdx.x * RPB + g; j < K; j += (int64_t)gridDim.x * RPB) {
-        const int s = rp[j], e = rp[j + 1];
-        if (s < 0 || e < s || (int64_t)e > nnz)
-            continue;
-        int run = tptr[j];
-        for (int k0 = s; k0 < e; k0 += G) {
-            const int k = k0 + l;
-            bool head = false;
-            int cc = 0;
-            if (k < e) {
-                cc = col[k];
-                head = (k == s) || ((col[k - 1] >> 6) != (cc >> 6));
-            }
-            const u64 bal = __ballot(head) & gmask;
-            if (head) {
-                const int t = run + mbcnt(bal);
-                const int tc = cc >> 6;
-                u64 m = 0;
-                for (int q = k; q < e; ++q) {
-                    const int c2 = col[q];
-                    if ((c2 >> 6) != tc)
-                        break;
-                    m |= 1ull << (c2 & 63);
-                }
-                tcol[t] = tc;
-                tmask[t] = m;
-            }
-            run += __popcll(bal);
-        }
-    }
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= nnz || !flag[k])
+        return;
+    const int c = col[k];
+    const int tc = c >> 6;
+    u64 m = 1ull << (c & 63);
+    for (int64_t q = k + 1; q < nnz && !flag[q]; ++q)
+        m |= 1ull << (col[q] & 63);
+    const int t = tix[k];
+    tcol[t] = tc;
+    tmask[t] = m;
 }
 
-template <int G>
-static void btiles_g(int64_t K, int64_t N, int64_t nnz, const int *rp, const int *col, int *tcnt, int *tptr, int *tcol,
-                     u64 *tmask, void *scan_ws, DevStats *st, int num_cus, hipStream_t s)
+__global__ __launch_bounds__(256) void k_btptr(int64_t K, int64_t nnz, const int *__restrict__ rp, const int *__restrict__ tix,
+                                               int *__restrict__ tptr, DevStats *st)
 {
-    constexpr int RPB = 256 / G;
-    int64_t blocks = (K + RPB - 1) / RPB;
-    const int64_t cap = (int64_t)num_cus * 32;
-    if (blocks > cap)
-        blocks = cap;
-    if (blocks < 1)
-        blocks = 1;
-    hipLaunchKernelGGL((k_btile_count<G>), dim3((unsigned)blocks), dim3(256), 0, s, K, N, nnz, rp, col, tcnt, st);
-    (void)scan_excl<int, int>(tcnt, K, tptr, scan_ws, nullptr, s);
-    hipLaunchKernelGGL((k_btile_fill<G>), dim3((unsigned)blocks), dim3(256), 0, s, K, nnz, rp, col, tptr, tcol, tmask);
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j > K)
+        return;
+    // a row pointer outside [0, nnz] has already raised ERR_ROWPTR in k_bmark; keep
+    // the read in bounds until the host sees the error
+    int r = rp[j];
+    r = r < 0 ? 0 : (r > nnz ? (int)nnz : r);
+    tptr[j] = tix[r];
+    if (j == K)
+        st->ntiles_b = tix[r];
 }
 
-__global__ void k_copy_ntiles(const int *tptr, int64_t K, DevStats *st) { st->ntiles_b = tptr[K]; }
+size_t btiles_ws_bytes(int64_t nnz) { return (size_t)(2 * nnz + 2) * sizeof(int); }
 
-hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const int *col, int *tcnt, int *tptr,
+hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const int *col, int *ws, int *tptr,
                          int *tcol, u64 *tmask, void *scan_ws, DevStats *st, int num_cus, hipStream_t s)
 {
-    const int64_t avg = K > 0 ? nnz / K : 0;
-    if (avg <= 12)
-        btiles_g<8>(K, N, nnz, rp, col, tcnt, tptr, tcol, tmask, scan_ws, st, num_cus, s);
-    else if (avg <= 24)
-        btiles_g<16>(K, N, nnz, rp, col, tcnt, tptr, tcol, tmask, scan_ws, st, num_cus, s);
-    else if (avg <= 48)
-        btiles_g<32>(K, N, nnz, rp, col, tcnt, tptr, tcol, tmask, scan_ws, st, num_cus, s);
-    else
-        btiles_g<64>(K, N, nnz, rp, col, tcnt, tptr, tcol, tmask, scan_ws, st, num_cus, s);
-    hipLaunchKernelGGL(k_copy_ntiles, dim3(1), dim3(1), 0, s, tptr, K, st);
+    (void)num_cus;
+    int *flag = ws;           // nnz + 1
+    int *tix = ws + nnz + 1;  // nnz + 1
+    const unsigned gk = (unsigned)((K + 256) / 256);
+    const unsigned ge = (unsigned)((nnz + 255) / 256 > 0 ? (nnz + 255) / 256 : 1);
+    hipError_t e = launch_fill_zero(flag, (size_t)(nnz + 1) * sizeof(int), s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_bmark, dim3(gk), dim3(256), 0, s, K, nnz, rp, flag, st);
+    hipLaunchKernelGGL(k_bhead, dim3(ge), dim3(256), 0, s, nnz, N, col, flag, st);
+    e = scan_excl<int, int>(flag, nnz, tix, scan_ws, nullptr, s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_bfill, dim3(ge), dim3(256), 0, s, nnz, col, flag, tix, tcol, tmask);
+    hipLaunchKernelGGL(k_btptr, dim3(gk), dim3(256), 0, s, K, nnz, rp, tix, tptr, st);
     return hipGetLastError();
 }
 
@@ -322,13 +311,15 @@ hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const
 // One group of G lanes per row, one row per group (no grid-stride loop, so the latency
 // of the dependent loads Acol -> Brp/tptr -> tcol is paid once per row in parallel).
 // Totals are summed per bin by k_bin; only the rare error bits use a global atomic.
+constexpr int LONG_A = 2048; // rows of A longer than this are analysed by a whole workgroup
+
 template <int G>
 __global__ __launch_bounds__(256) void k_row_info(int64_t M, int64_t K, int64_t nnzA, const int *__restrict__ Arp,
                                                   const int *__restrict__ Acol, const int *__restrict__ Brp,
                                                   const int *__restrict__ tptr, const int *__restrict__ tcol,
                                                   long long *__restrict__ rprod, int *__restrict__ rtp,
                                                   int *__restrict__ rmin, int *__restrict__ rmax, int *__restrict__ rnnz,
-                                                  DevStats *st)
+                                                  int *__restrict__ longrows, DevStats *st)
 {
     constexpr int RPB = 256 / G;
     const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
@@ -341,6 +332,10 @@ __global__ __launch_bounds__(256) void k_row_info(int64_t M, int64_t K, int64_t 
     int tp = 0, mn = INT_MAX, mx = -1;
     if (s < 0 || e < s || (int64_t)e > nnzA) {
         bad |= ERR_ROWPTR;
+    } else if (e - s > LONG_A) {
+        if (l == 0)
+            longrows[atomicAdd(&st->nlong, 1)] = (int)i;
+        return;
     } else {
         for (int k = s + l; k < e; k += G) {
             const int j = Acol[k];
@@ -375,16 +370,86 @@ __global__ __launch_bounds__(256) void k_row_info(int64_t M, int64_t K, int64_t 
         atomicOr(&st->err, bad);
 }
 
+// The rows k_row_info set aside (more than LONG_A entries): one 1024-lane workgroup per
+// row, grid-stride over the device-side list.
+__global__ __launch_bounds__(1024) void k_row_info_long(int64_t K, const int *__restrict__ longrows,
+                                                        const int *__restrict__ Arp, const int *__restrict__ Acol,
+                                                        const int *__restrict__ Brp, const int *__restrict__ tptr,
+                                                        const int *__restrict__ tcol, long long *__restrict__ rprod,
+                                                        int *__restrict__ rtp, int *__restrict__ rmin,
+                                                        int *__restrict__ rmax, int *__restrict__ rnnz, DevStats *st)
+{
+    __shared__ long long s_p[16];
+    __shared__ int s_tp[16], s_mn[16], s_mx[16], s_bad[16];
+    const int n = st->nlong;
+    const int w = threadIdx.x >> 6;
+    for (int idx = blockIdx.x; idx < n; idx += gridDim.x) {
+        const int i = longrows[idx];
+        const int s = Arp[i], e = Arp[i + 1];
+        long long p = 0;
+        int tp = 0, mn = INT_MAX, mx = -1, bad = 0;
+        for (int k = s + threadIdx.x; k < e; k += 1024) {
+            const int j = Acol[k];
+            if ((unsigned)j >= (unsigned long long)K) {
+                bad |= ERR_ACOL;
+                continue;
+            }
+            const int b0 = Brp[j], b1 = Brp[j + 1];
+            const int t0 = tptr[j], t1 = tptr[j + 1];
+            if (b1 > b0 && t1 > t0) {
+                p += b1 - b0;
+                tp += t1 - t0;
+                const int f = tcol[t0], z = tcol[t1 - 1];
+                mn = f < mn ? f : mn;
+                mx = z > mx ? z : mx;
+            }
+        }
+        p = group_sum<64>(p, (long long *)nullptr);
+        tp = group_sum<64>(tp, (int *)nullptr);
+        mn = group_min<64>(mn);
+        mx = group_max<64>(mx);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+            bad |= __shfl_xor(bad, o, 64);
+        if ((threadIdx.x & 63) == 0) {
+            s_p[w] = p;
+            s_tp[w] = tp;
+            s_mn[w] = mn;
+            s_mx[w] = mx;
+            s_bad[w] = bad;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int k = 1; k < 16; ++k) {
+                p += s_p[k];
+                tp += s_tp[k];
+                mn = s_mn[k] < mn ? s_mn[k] : mn;
+                mx = s_mx[k] > mx ? s_mx[k] : mx;
+                bad |= s_bad[k];
+            }
+            rprod[i] = p;
+            rtp[i] = tp;
+            rmin[i] = mn;
+            rmax[i] = mx;
+            if (p == 0)
+                rnnz[i] = 0;
+            if (bad)
+                atomicOr(&st->err, bad);
+        }
+        __syncthreads();
+    }
+}
+
 hipError_t launch_row_info(int64_t M, int64_t K, int64_t nnzA, const int *Arp, const int *Acol, const int *Brp,
                            const int *tptr, const int *tcol, long long *rprod, int *rtp, int *rmin, int *rmax, int *rnnz,
-                           DevStats *st, int avg_a, hipStream_t s)
+                           int *longrows, DevStats *st, int avg_a, int num_cus, hipStream_t s)
 {
 #define ROW_INFO(G)                                                                                                    \
     {                                                                                                                  \
         constexpr int RPB = 256 / G;                                                                                   \
         const int64_t blocks = (M + RPB - 1) / RPB;                                                                    \
         hipLaunchKernelGGL((k_row_info<G>), dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(256), 0, s, M, K, nnzA,    \
-                           Arp, Acol, Brp, tptr, tcol, rprod, rtp, rmin, rmax, rnnz, st);                              \
+                           Arp, Acol, Brp, tptr, tcol, rprod, rtp, rmin, rmax, rnnz, longrows, st);                    \
     }
     if (avg_a <= 12)
         ROW_INFO(8)
@@ -395,6 +460,8 @@ hipError_t launch_row_info(int64_t M, int64_t K, int64_t nnzA, const int *Arp, c
     else
         ROW_INFO(64)
 #undef ROW_INFO
+    hipLaunchKernelGGL(k_row_info_long, dim3((unsigned)num_cus), dim3(1024), 0, s, K, longrows, Arp, Acol, Brp, tptr,
+                       tcol, rprod, rtp, rmin, rmax, rnnz, st);
     return hipGetLastError();
 }
 

@@ -321,6 +321,18 @@ class Context:
         val = self.download(c.val, c.nnz, np.float32 if c.val32 else np.float64)
         return HostMatrix(int(c.M), int(c.N), rp, col, val)
 
+    def fetch_rows(self, c: Csr, r0: int, r1: int) -> HostMatrix:
+        """Rows [r0, r1) of a device result as a matrix of r1 - r0 rows (row pointers
+        rebased to 0), for checking large results a panel at a time."""
+        esz = 8 if c.rowptr64 else 4
+        rp = self.download(c.rowptr + r0 * esz, r1 - r0 + 1, np.int64 if c.rowptr64 else np.int32)
+        rp = rp.astype(np.int64)
+        base, end = int(rp[0]), int(rp[-1])
+        col = self.download(c.col + base * 4, end - base, np.int32)
+        vsz = 4 if c.val32 else 8
+        val = self.download(c.val + base * vsz, end - base, np.float32 if c.val32 else np.float64)
+        return HostMatrix(r1 - r0, int(c.N), rp - base, col, val)
+
     def spgemm(self, A: HostMatrix, B: HostMatrix, **opts) -> tuple[HostMatrix, dict]:
         """Upload A and B, compute C = A * B, download C, free everything."""
         val32 = int(opts.get("precision", HSMU_FP64)) == HSMU_FP32_COMP
