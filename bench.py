@@ -41,6 +41,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32c", action="store_true", help="skip the compensated-FP32 form")
     ap.add_argument("--opts", default="", help="hsmu_opts overrides, e.g. products_per_lane=16,b_lanes=32")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="size of the device-to-device copy probe (achievable HBM bandwidth; 0 = skip)")
@@ -104,6 +105,28 @@ def main():
         f"stages {stage}; numeric bins {st0['num_bin_rows']}; dominant bin {dom}: {dom_ms:.3f} ms, "
         f"{achieved:.0f} GB/s")
 
+    # compensated-FP32 form on the same workload (values rounded to FP32)
+    fp32c = None
+    if not args.no_fp32c:
+        import numpy as np
+        A32 = hsmu.HostMatrix(A.M, A.N, A.rowptr, A.col, A.val.astype(np.float32))
+        dA32 = ctx.matrix(A32, val32=True)
+        o32 = hsmu.default_opts(precision=hsmu.HSMU_FP32_COMP)
+        for _ in range(args.warmup):
+            c, _ = ctx.spgemm_device(dA32, dA32, o32)
+            ctx.free_result(c)
+        hsmu.lib.hsmu_sync(ctx.handle)
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            c, st32 = ctx.spgemm_device(dA32, dA32, o32)
+            ctx.free_result(c)
+        hsmu.lib.hsmu_sync(ctx.handle)
+        ms32 = (time.perf_counter() - t1) * 1e3 / args.steps
+        fp32c = {"value": round(2.0 * products / (ms32 * 1e6), 3), "unit": "GFLOP/s", "ms_per_step": round(ms32, 4),
+                 "speedup_vs_fp64": round(ms_step / ms32, 3), "numeric_ms": round(st32["ms_form_cval"], 4)}
+        log(f"fp32 compensated: {fp32c}")
+        dA32.free()
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -145,6 +168,7 @@ def main():
                      "kernel_ms": round(dom_ms, 4), "alg_bytes_per_launch": dom_bytes,
                      "copy_kernel_GBps": round(copy_gbs, 1) if copy_gbs else None},
         "cpu_baseline": cpu,
+        "forms": {"fp32_compensated": fp32c},
     }
     print(json.dumps(out), flush=True)
     dA.free()

@@ -44,7 +44,7 @@ DEVI void stage_row(const Operands &op, int a0, int nA, int *t0, int *tlen, int 
             const int bb = op.Brp[j];
             b0[k] = bb;
             blen[k] = op.Brp[j + 1] - bb;
-            av[k] = op.Aval[a0 + k];
+            av[k] = op.val32 ? (double)((const float *)op.Aval)[a0 + k] : ((const double *)op.Aval)[a0 + k];
         }
     }
 }
@@ -82,13 +82,85 @@ DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, u
     }
 }
 
+// ---- numeric forms ---------------------------------------------------------------------
+//
+// A slot is 8 bytes in both forms, so the LDS layout and the bins are the same.
+// PrecF64 (plain): FP64 values, slot = one double, ds_add_f64 per product.
+// PrecF32C (compensated FP32): FP32 values; the product is split exactly into
+// p = fl(a*b) and e = fma(a, b, -p) (TwoProduct); the slot holds a float pair (s, c):
+// p is added to s by a compare-and-swap loop (so the stored sum is the VALU's round-to-
+// nearest-even fl(s + p)), the exact rounding error of that addition is recomputed
+// (TwoSum, Knuth) and err + e go into c with a plain float atomic add.  The result is
+// fl(s + c): c carries every rounding error of the products and of the additions into
+// s, so the sum is accurate to about one FP32 rounding of the result plus O(n u^2) of
+// sum |a*b| (u = 2^-24).
+struct PrecF64 {
+    typedef double V;
+    static DEVI void add(double *vals, unsigned slot, double a, V b) { atomicAdd(&vals[slot], a * b); }
+    static DEVI V result(const double *vals, unsigned slot) { return vals[slot]; }
+    // global-memory accumulation (rows outside LDS): straight into C
+    static DEVI void gadd(V *Cval, float *, long long q, double a, V b) { unsafeAtomicAdd(&Cval[q], a * b); }
+};
+
+struct PrecF32C {
+    typedef float V;
+    // s += p atomically with the sum rounded by the VALU (round to nearest even): a
+    // compare-and-swap loop, because the rounding of the memory units' float atomic add
+    // does not match, and the TwoSum error below must describe the addition actually
+    // stored.  The error of that addition plus e then goes into c, whose own rounding is
+    // second order (|c| ~ u |s|).
+    template <bool LDS> static DEVI void two_add(float *s, float *c, float p, float e)
+    {
+        unsigned *si = (unsigned *)s;
+        float old = *s;
+        float sum;
+        while (true) {
+            sum = old + p;
+            const unsigned want = __float_as_uint(old);
+            const unsigned prev = atomicCAS(si, want, __float_as_uint(sum));
+            if (prev == want)
+                break;
+            old = __uint_as_float(prev);
+        }
+        const float bb = sum - old;
+        const float err = (old - (sum - bb)) + (p - bb);
+        if constexpr (LDS)
+            atomicAdd(c, err + e);
+        else
+            unsafeAtomicAdd(c, err + e);
+    }
+    static DEVI void add(double *vals, unsigned slot, double a, V b)
+    {
+        const float af = (float)a;
+        const float pr = __fmul_rn(af, b);
+        const float e = __builtin_fmaf(af, b, -pr);
+        float *f = (float *)vals + 2 * slot;
+        two_add<true>(f, f + 1, pr, e);
+    }
+    static DEVI V result(const double *vals, unsigned slot)
+    {
+        const float *f = (const float *)vals + 2 * slot;
+        return f[0] + f[1];
+    }
+    // global-memory accumulation: s in C, c in a compensation array of nnz(C) floats
+    static DEVI void gadd(V *Cval, float *comp, long long q, double a, V b)
+    {
+        const float af = (float)a;
+        const float pr = __fmul_rn(af, b);
+        const float e = __builtin_fmaf(af, b, -pr);
+        two_add<false>(&Cval[q], &comp[q], pr, e);
+    }
+};
+
 // Add every product a_ij * b_jc of the row into its slot: sub-groups of 2^bshift lanes
 // take one entry each and walk B_j, UNROLL elements per lane per iteration with all
 // loads issued before the ds_add_f64s.
-template <int G, bool DENSE>
+template <int G, bool DENSE, typename P>
 DEVI void add_products(const Operands &op, const int *b0, const int *blen, const double *av, int nA, const u64 *dm,
                        const unsigned *doff, double *vals, int mn, int bshift)
 {
+    typedef typename P::V V;
+    const V *__restrict__ Bval = (const V *)op.Bval;
     const int l = threadIdx.x & (G - 1);
     const int SG = 1 << bshift, sl = l & (SG - 1), NSG = G >> bshift;
     for (int e = l >> bshift; e < nA; e += NSG) {
@@ -96,15 +168,15 @@ DEVI void add_products(const Operands &op, const int *b0, const int *blen, const
         const double a = av[e];
         for (int k = kb + sl; k < ke; k += UNROLL * SG) {
             int c[UNROLL];
-            double v[UNROLL];
+            V v[UNROLL];
 #pragma unroll
             for (int u = 0; u < UNROLL; ++u) {
                 const int ku = k + u * SG;
                 c[u] = -1;
-                v[u] = 0.0;
+                v[u] = 0;
                 if (ku < ke) {
                     c[u] = op.Bcol[ku];
-                    v[u] = op.Bval[ku];
+                    v[u] = Bval[ku];
                 }
             }
 #pragma unroll
@@ -117,7 +189,7 @@ DEVI void add_products(const Operands &op, const int *b0, const int *blen, const
                         const int t = (c[u] >> 6) - mn;
                         slot = doff[t] + (unsigned)__popcll(dm[t] & below(c[u] & 63));
                     }
-                    atomicAdd(&vals[slot], a * v[u]);
+                    P::add(vals, slot, a, v[u]);
                 }
             }
         }
@@ -229,11 +301,11 @@ __global__ __launch_bounds__(1024) void k_symbolic_global(const int *__restrict_
 // and the row of C is written in column order: for G >= 64 one wave per tile, lane b
 // writing column 64*(rmin+t)+b if present (coalesced); for G < 64 one lane per tile
 // walking its set bits.
-template <int G>
+template <int G, typename P>
 __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int *__restrict__ rows,
                                                                       const int *__restrict__ count_p, int slice_bytes,
                                                                       Operands op, const long long *__restrict__ crp,
-                                                                      int *__restrict__ Ccol, double *__restrict__ Cval,
+                                                                      int *__restrict__ Ccol, typename P::V *__restrict__ Cval,
                                                                       int bshift, int tshift, int *err)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
@@ -308,9 +380,9 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
         __syncthreads();
         if (row >= 0 && ok) {
             if (dense)
-                add_products<G, true>(op, b0, blen, av, nA, dm, doff, vals, mn, bshift);
+                add_products<G, true, P>(op, b0, blen, av, nA, dm, doff, vals, mn, bshift);
             else
-                add_products<G, false>(op, b0, blen, av, nA, dm, doff, vals, mn, bshift);
+                add_products<G, false, P>(op, b0, blen, av, nA, dm, doff, vals, mn, bshift);
         }
         __syncthreads();
         if (row >= 0 && ok) {
@@ -322,7 +394,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
                     if ((m >> lane) & 1ull) {
                         const unsigned pos = doff[t] + (unsigned)__popcll(m & below(lane));
                         Ccol[cb + pos] = ((mn + t) << 6) + lane;
-                        Cval[cb + pos] = vals[dense ? (t << 6) + lane : (int)pos];
+                        Cval[cb + pos] = P::result(vals, dense ? (t << 6) + lane : pos);
                     }
                 }
             } else {
@@ -334,14 +406,14 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
                         const int b = __ffsll((unsigned long long)m) - 1;
                         Ccol[cb + pos] = c0 + b;
                         if (dense)
-                            Cval[cb + pos] = vals[(t << 6) + b];
+                            Cval[cb + pos] = P::result(vals, (t << 6) + b);
                         ++pos;
                         m &= m - 1;
                     }
                 }
                 if (!dense)
                     for (int q = l; q < n; q += G)
-                        Cval[cb + q] = vals[q];
+                        Cval[cb + q] = P::result(vals, q);
             }
         }
         __syncthreads();
@@ -349,12 +421,18 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
 }
 
 // Rows that do not fit LDS: directory (masks + offsets) in a per-workgroup global
-// workspace, FP64 accumulation with global atomics straight into C's value array.
+// workspace, accumulation with global atomics straight into C's value array (FP32C: the
+// compensation terms go to comp[], folded into C at the end of the row).
+template <typename P>
 __global__ __launch_bounds__(1024) void k_numeric_global(const int *__restrict__ rows, const int *__restrict__ count_p,
                                                          Operands op, const long long *__restrict__ crp,
-                                                         int *__restrict__ Ccol, double *Cval, int bshift, int tshift,
-                                                         u64 *gws, long long gws_stride, long long gws_tiles, int *err)
+                                                         int *__restrict__ Ccol, typename P::V *Cval, float *comp,
+                                                         int bshift, int tshift, u64 *gws, long long gws_stride,
+                                                         long long gws_tiles, int *err)
 {
+    typedef typename P::V V;
+    const V *__restrict__ Aval = (const V *)op.Aval;
+    const V *__restrict__ Bval = (const V *)op.Bval;
     __shared__ unsigned red[16];
     __shared__ int ok_s;
     const int l = threadIdx.x;
@@ -370,8 +448,11 @@ __global__ __launch_bounds__(1024) void k_numeric_global(const int *__restrict__
         const int n = (int)(crp[row + 1] - cb);
         for (int t = l; t < S; t += 1024)
             dm[t] = 0;
-        for (int q = l; q < n; q += 1024)
-            Cval[cb + q] = 0.0;
+        for (int q = l; q < n; q += 1024) {
+            Cval[cb + q] = 0;
+            if (comp)
+                comp[cb + q] = 0.0f;
+        }
         block_sync_global();
         {
             const int SG = 1 << tshift, sl = l & (SG - 1), sg = l >> tshift, NSG = 1024 >> tshift;
@@ -407,13 +488,13 @@ __global__ __launch_bounds__(1024) void k_numeric_global(const int *__restrict__
             const int a1 = op.Arp[row + 1];
             for (int e = op.Arp[row] + sg; e < a1; e += NSG) {
                 const int j = op.Acol[e];
-                const double a = op.Aval[e];
+                const double a = (double)Aval[e];
                 const int b1 = op.Brp[j + 1];
                 for (int k = op.Brp[j] + sl; k < b1; k += SG) {
                     const int c = op.Bcol[k];
                     const int t = (c >> 6) - mn;
                     const unsigned slot = doff[t] + (unsigned)__popcll(dm[t] & below(c & 63));
-                    unsafeAtomicAdd(&Cval[cb + slot], a * op.Bval[k]);
+                    P::gadd(Cval, comp, cb + slot, a, Bval[k]);
                 }
             }
             const int w = l >> 6, lane = l & 63;
@@ -423,6 +504,11 @@ __global__ __launch_bounds__(1024) void k_numeric_global(const int *__restrict__
                     const unsigned pos = doff[t] + (unsigned)__popcll(m & below(lane));
                     Ccol[cb + pos] = ((mn + t) << 6) + lane;
                 }
+            }
+            if (comp) {
+                block_sync_global();
+                for (int q = l; q < n; q += 1024)
+                    Cval[cb + q] = (V)(Cval[cb + q] + comp[cb + q]);
             }
         }
         block_sync_global();
@@ -473,9 +559,9 @@ hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int c
     return hipGetLastError();
 }
 
-template <int G>
+template <int G, typename P>
 static void numeric_g(const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
-                      const long long *crp, int *Ccol, double *Cval, int bshift, int tshift, int *err, hipStream_t s)
+                      const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int *err, hipStream_t s)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
@@ -484,34 +570,49 @@ static void numeric_g(const int *rows, const int *count_dev, int count, int max_
     if (blocks > (1 << 20))
         blocks = 1 << 20;
     const int lg = __builtin_ctz(G);
-    hipLaunchKernelGGL((k_numeric_lds<G>), dim3((unsigned)blocks), dim3(BS), (size_t)slice * RPB, s, rows, count_dev,
-                       slice, op, crp, Ccol, Cval, bshift > lg ? lg : bshift, tshift > lg ? lg : tshift, err);
+    hipLaunchKernelGGL((k_numeric_lds<G, P>), dim3((unsigned)blocks), dim3(BS), (size_t)slice * RPB, s, rows, count_dev,
+                       slice, op, crp, Ccol, (typename P::V *)Cval, bshift > lg ? lg : bshift, tshift > lg ? lg : tshift,
+                       err);
 }
 
-hipError_t launch_numeric(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
-                          const long long *crp, int *Ccol, double *Cval, int bshift, int tshift, u64 *gws,
-                          long long gws_stride, long long gws_tiles, int gws_blocks, int *err, hipStream_t s)
+template <typename P>
+static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
+                            const long long *crp, int *Ccol, void *Cval, float *comp, int bshift, int tshift, u64 *gws,
+                            long long gws_stride, long long gws_tiles, int gws_blocks, int *err, hipStream_t s)
 {
-    if (count <= 0)
-        return hipSuccess;
     switch (bin) {
-    case 0: numeric_g<8>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 1: numeric_g<16>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 2: numeric_g<32>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 3: numeric_g<64>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 4: numeric_g<128>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 5: numeric_g<256>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 6: numeric_g<512>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 7: numeric_g<1024>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 0: numeric_g<8, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 1: numeric_g<16, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 2: numeric_g<32, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 3: numeric_g<64, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 4: numeric_g<128, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 5: numeric_g<256, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 6: numeric_g<512, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 7: numeric_g<1024, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
     case BIN_GLOBAL: {
         const int blocks = count < gws_blocks ? count : gws_blocks;
-        hipLaunchKernelGGL(k_numeric_global, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, crp, Ccol, Cval,
-                           bshift > 10 ? 10 : bshift, tshift > 10 ? 10 : tshift, gws, gws_stride, gws_tiles, err);
+        hipLaunchKernelGGL(k_numeric_global<P>, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, crp, Ccol,
+                           (typename P::V *)Cval, comp, bshift > 10 ? 10 : bshift, tshift > 10 ? 10 : tshift, gws,
+                           gws_stride, gws_tiles, err);
         break;
     }
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count, int max_need,
+                          const Operands &op, const long long *crp, int *Ccol, void *Cval, float *comp, int bshift,
+                          int tshift, u64 *gws, long long gws_stride, long long gws_tiles, int gws_blocks, int *err,
+                          hipStream_t s)
+{
+    if (count <= 0)
+        return hipSuccess;
+    if (fp32c)
+        return numeric_p<PrecF32C>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, comp, bshift, tshift,
+                                   gws, gws_stride, gws_tiles, gws_blocks, err, s);
+    return numeric_p<PrecF64>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, nullptr, bshift, tshift, gws,
+                              gws_stride, gws_tiles, gws_blocks, err, s);
 }
 
 hipError_t configure_kernels(int lds_max)
@@ -525,8 +626,12 @@ hipError_t configure_kernels(int lds_max)
     }
     CFG(k_symbolic_lds<8>) CFG(k_symbolic_lds<16>) CFG(k_symbolic_lds<32>) CFG(k_symbolic_lds<64>)
     CFG(k_symbolic_lds<128>) CFG(k_symbolic_lds<256>) CFG(k_symbolic_lds<512>) CFG(k_symbolic_lds<1024>)
-    CFG(k_numeric_lds<8>) CFG(k_numeric_lds<16>) CFG(k_numeric_lds<32>) CFG(k_numeric_lds<64>)
-    CFG(k_numeric_lds<128>) CFG(k_numeric_lds<256>) CFG(k_numeric_lds<512>) CFG(k_numeric_lds<1024>)
+#define CFGP(P)                                                                                                        \
+    CFG((k_numeric_lds<8, P>)) CFG((k_numeric_lds<16, P>)) CFG((k_numeric_lds<32, P>)) CFG((k_numeric_lds<64, P>))     \
+    CFG((k_numeric_lds<128, P>)) CFG((k_numeric_lds<256, P>)) CFG((k_numeric_lds<512, P>))                           \
+    CFG((k_numeric_lds<1024, P>))
+    CFGP(PrecF64) CFGP(PrecF32C)
+#undef CFGP
 #undef CFG
     return e;
 }

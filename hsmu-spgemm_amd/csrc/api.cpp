@@ -414,7 +414,7 @@ float elapsed(hipEvent_t a, hipEvent_t b)
 }
 
 // C with M rows and no entries.
-int empty_product(hsmu_ctx *c, int64_t M, int64_t N, hsmu_csr *C, bool rp64, bool plain, hsmu_stats *st)
+int empty_product(hsmu_ctx *c, int64_t M, int64_t N, hsmu_csr *C, bool rp64, bool plain, bool val32, hsmu_stats *st)
 {
     const size_t rpb = (size_t)(M + 1) * (rp64 ? 8 : 4);
     void *rp = out_alloc(c, rpb, plain);
@@ -429,7 +429,7 @@ int empty_product(hsmu_ctx *c, int64_t M, int64_t N, hsmu_csr *C, bool rp64, boo
     C->col = nullptr;
     C->val = nullptr;
     C->rowptr64 = rp64;
-    C->val32 = 0;
+    C->val32 = val32;
     if (st)
         memset(st, 0, sizeof(*st));
     return HSMU_OK;
@@ -516,11 +516,13 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
         opts = *opts_in;
     else
         hsmu_opts_default(&opts);
-    if (opts.precision != HSMU_FP64)
-        return fail(HSMU_ERR_UNSUPPORTED, "precision %d not built", opts.precision);
-    int rc = check_input(A, "A", false);
+    if (opts.precision != HSMU_FP64 && opts.precision != HSMU_FP32_COMP)
+        return fail(HSMU_ERR_INVALID, "unknown precision %d", opts.precision);
+    const bool fp32 = opts.precision == HSMU_FP32_COMP;
+    const double vsz = fp32 ? 4.0 : 8.0; // bytes per value
+    int rc = check_input(A, "A", fp32);
     if (!rc)
-        rc = check_input(B, "B", false);
+        rc = check_input(B, "B", fp32);
     if (rc)
         return rc;
     if (A->N != B->M)
@@ -534,7 +536,7 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     if (stats)
         memset(stats, 0, sizeof(*stats));
     if (M == 0 || A->nnz == 0 || B->nnz == 0)
-        return empty_product(c, M, N, C, rp64, plain, stats);
+        return empty_product(c, M, N, C, rp64, plain, fp32, stats);
 
     hipStream_t s = c->stream;
     const bool timing = opts.timing != 0;
@@ -626,10 +628,11 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     Operands op;
     op.Arp = (const int *)A->rowptr;
     op.Acol = A->col;
-    op.Aval = (const double *)A->val;
+    op.Aval = A->val;
     op.Brp = (const int *)B->rowptr;
     op.Bcol = B->col;
-    op.Bval = (const double *)B->val;
+    op.Bval = B->val;
+    op.val32 = fp32 ? 1 : 0;
     op.tptr = tptr;
     op.tcol = tcol;
     op.tmask = tmask;
@@ -695,7 +698,7 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     // ---- 4. allocate C ----
     const auto t_alloc0 = std::chrono::steady_clock::now();
     int *Ccol = (int *)out_alloc(c, (size_t)nnzc * sizeof(int), plain);
-    double *Cval = (double *)out_alloc(c, (size_t)nnzc * sizeof(double), plain);
+    void *Cval = out_alloc(c, (size_t)nnzc * (size_t)vsz, plain);
     void *Crp = rp64 ? (void *)crp : out_alloc(c, ((size_t)M + 1) * sizeof(int), plain);
     if (!Ccol || !Cval || !Crp) {
         out_free(c, Ccol);
@@ -719,10 +722,17 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
             if (rc)
                 return rc;
         }
+        // compensation terms of FP32C rows accumulated in global memory
+        float *comp = nullptr;
+        if (fp32 && b == BIN_GLOBAL) {
+            comp = ws.get<float>((size_t)nnzc);
+            if (!comp)
+                return fail(HSMU_ERR_NOMEM, "compensation scratch (nnz %lld)", nnzc);
+        }
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][0], s));
-        HIP_TRY(launch_numeric(b, binrows + (size_t)b * M, &dst->num.count[b], cnt, h2.num.max_need[b], op, crp, Ccol,
-                               Cval, bshift, tshift, gws, gws_stride, gws_tiles, gws_blocks, errp, s));
+        HIP_TRY(launch_numeric(b, fp32 ? 1 : 0, binrows + (size_t)b * M, &dst->num.count[b], cnt, h2.num.max_need[b], op,
+                               crp, Ccol, Cval, comp, bshift, tshift, gws, gws_stride, gws_tiles, gws_blocks, errp, s));
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][1], s));
     }
@@ -749,7 +759,7 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     C->col = Ccol;
     C->val = Cval;
     C->rowptr64 = rp64;
-    C->val32 = 0;
+    C->val32 = fp32 ? 1 : 0;
 
     if (stats) {
         hsmu_stats &S = *stats;
@@ -780,17 +790,17 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
             S.num_bin_rows[b] = h2.num.count[b];
             // gather-model bytes of the rows of this numeric bin: A row extent, A col+val,
             // B row extent per a_ij, one B (col, val) per product, C row pointer, C col+val
-            S.num_bin_bytes[b] = 4.0 * 2.0 * h2.num.count[b] + 12.0 * (double)h2.num.nnz_a[b] +
-                                 8.0 * (double)h2.num.nnz_a[b] + 12.0 * (double)h2.num.work[b] +
-                                 (rp64 ? 8.0 : 4.0) * h2.num.count[b] + 12.0 * (double)h2.num.nnz_c[b];
+            S.num_bin_bytes[b] = 4.0 * 2.0 * h2.num.count[b] + (4.0 + vsz) * (double)h2.num.nnz_a[b] +
+                                 8.0 * (double)h2.num.nnz_a[b] + (4.0 + vsz) * (double)h2.num.work[b] +
+                                 (rp64 ? 8.0 : 4.0) * h2.num.count[b] + (4.0 + vsz) * (double)h2.num.nnz_c[b];
             S.num_bin_products[b] = (int64_t)h2.num.work[b];
         }
         S.compress_rate = nnzc > 0 ? (double)S.products / (double)nnzc : 0.0;
         S.gflops = S.ms_time2 > 0 ? 2.0 * (double)S.products / (S.ms_time2 * 1e6) : 0.0;
         S.space_compressed_mask_mb =
             ((double)(K + 1) * 4.0 + (double)h2.ntiles_b * (4.0 + 8.0)) / (1024.0 * 1024.0);
-        S.alg_bytes = 4.0 * (double)(M + 1) + 20.0 * (double)A->nnz + 12.0 * (double)S.products +
-                      (rp64 ? 8.0 : 4.0) * (double)(M + 1) + 12.0 * (double)nnzc;
+        S.alg_bytes = 4.0 * (double)(M + 1) + (12.0 + vsz) * (double)A->nnz + (4.0 + vsz) * (double)S.products +
+                      (rp64 ? 8.0 : 4.0) * (double)(M + 1) + (4.0 + vsz) * (double)nnzc;
     }
     return HSMU_OK;
 }

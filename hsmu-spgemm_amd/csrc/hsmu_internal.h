@@ -147,9 +147,10 @@ hipError_t launch_bin(int numeric, int64_t M, const int *Arp, const long long *r
 
 struct Operands {
     const int *Arp, *Acol;
-    const double *Aval;
+    const void *Aval; // double, or float when val32
     const int *Brp, *Bcol;
-    const double *Bval;
+    const void *Bval;
+    int val32;
     const int *tptr, *tcol;
     const unsigned long long *tmask;
     const int *rmin, *rmax;
@@ -172,10 +173,13 @@ hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int c
 hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *scan_ws,
                                long long *total_copy, hipStream_t s);
 
-hipError_t launch_numeric(int bin, const int *rows, const int *count_dev, int count, int max_need,
-                          const Operands &op, const long long *crp, int *Ccol, double *Cval,
-                          int bshift, int tshift, unsigned long long *gws, long long gws_stride,
-                          long long gws_tiles, int gws_blocks, int *err, hipStream_t s);
+// Numeric phase of one bin.  fp32c = 0: FP64 values (Cval double); fp32c = 1: compensated
+// FP32 (Cval float; comp = nnz(C) floats of scratch, needed only for BIN_GLOBAL).
+hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count,
+                          int max_need, const Operands &op, const long long *crp, int *Ccol,
+                          void *Cval, float *comp, int bshift, int tshift, unsigned long long *gws,
+                          long long gws_stride, long long gws_tiles, int gws_blocks, int *err,
+                          hipStream_t s);
 
 hipError_t launch_rowptr_i64_to_i32(const long long *in, int64_t n, int *out, hipStream_t s);
 

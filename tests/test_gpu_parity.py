@@ -280,3 +280,67 @@ def test_driver_prints_reference_lines(tmp_path):
                  "compress_rate =", "the gflops is"]:
         assert line in out.stdout, (line, out.stdout)
     assert os.path.exists(tmp_path / "NHC_MI355X_result.csv")
+
+
+def as_fp32(m):
+    """The same matrix with values rounded to FP32 (kept in float64 for the oracle)."""
+    return hsmu.HostMatrix(m.M, m.N, m.rowptr, m.col, m.val.astype(np.float32).astype(np.float64))
+
+
+def check_fp32c(ctx, A, B, name, **opts):
+    """Compensated FP32: structure bit-exact; |c - c_ref| <= 2^-23 |c_ref| + 1e-10 sum|a*b|,
+    c_ref the FP64 oracle on the same FP32-rounded inputs."""
+    A32, B32 = as_fp32(A), as_fp32(B)
+    note(f"fp32c {name}: A {A.M}x{A.N} nnz {A.nnz}, opts {opts}")
+    Cm, st = ctx.spgemm(A32, B32, precision=hsmu.HSMU_FP32_COMP, **opts)
+    assert Cm.val.dtype == np.float32
+    ref = oracle.spgemm(A32, B32)
+    assert np.array_equal(np.asarray(Cm.rowptr, np.int64), ref.rowptr) and np.array_equal(Cm.col, ref.col)
+    got = Cm.val.astype(np.float64)
+    err = np.abs(got - ref.val)
+    bound = 2.0 ** -23 * np.abs(ref.val) + 1e-10 * ref.absval
+    assert np.all(err <= bound), f"max excess {np.max(err - bound):.3e}"
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rel = np.where(np.abs(ref.val) > 0, err / np.abs(ref.val), 0.0)
+    note(f"fp32c {name}: max rel err {rel.max(initial=0):.2e}, median {np.median(rel) if len(rel) else 0:.2e}")
+    return rel
+
+
+@pytest.mark.parametrize("case", ["laplace2d", "banded", "random", "rmat", "zipf_aat"])
+def test_fp32_compensated(ctx, case):
+    if case == "laplace2d":
+        A, B = hsmu.gen_laplace2d(64), None
+    elif case == "banded":
+        A, B = hsmu.gen_banded(6000, 40, 88, 400, seed=3), None
+    elif case == "random":
+        A, B = hsmu.gen_random(800, 900, 30, 5), hsmu.gen_random(900, 30000, 60, 6)
+    elif case == "rmat":
+        A, B = hsmu.gen_rmat(12, 16, seed=2), None
+    else:
+        A = hsmu.gen_zipf(3000, 1500, 8, 1.2, seed=4)
+        B = hsmu.transpose(A)
+    rel = check_fp32c(ctx, A, A if B is None else B, case)
+    assert np.median(rel) < 1e-6
+
+
+@pytest.mark.parametrize("opts", [dict(force_global=1), dict(lds_row_max=512), dict(rebuild_dirs=1)])
+def test_fp32_compensated_fallbacks(ctx, opts):
+    A = hsmu.gen_banded(3000, 20, 90, 700, seed=9)
+    check_fp32c(ctx, A, A, "banded fallback", **opts)
+
+
+def test_fp32_compensated_beats_plain_fp32_sum(ctx):
+    """Long sums of mixed-sign terms: the compensated result keeps FP64-level accuracy
+    where a plain FP32 sum of the same FP32 products would lose about 3 digits."""
+    n = 4000
+    rng = np.random.default_rng(11)
+    a = rng.uniform(-1, 1, n).astype(np.float32).astype(np.float64)
+    b = (rng.uniform(-1, 1, n) * 10.0 ** rng.integers(-3, 4, n)).astype(np.float32).astype(np.float64)
+    A = hsmu.HostMatrix(1, n, np.array([0, n], np.int32), np.arange(n, dtype=np.int32), a)
+    B = hsmu.HostMatrix(n, 1, np.arange(n + 1, dtype=np.int32), np.zeros(n, np.int32), b)
+    rel = check_fp32c(ctx, A, B, "long dot product")
+    exact = float(np.dot(a, b))
+    naive = np.float32(0)
+    for x, y in zip(a.astype(np.float32), b.astype(np.float32)):
+        naive = np.float32(naive + x * y)
+    assert rel[0] < 1e-6 and rel[0] * 10 < abs(float(naive) - exact) / abs(exact)
