@@ -203,17 +203,22 @@ DEVI void add_products(const Operands &op, const int *b0, const int *blen, const
 template <int G>
 __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const int *__restrict__ rows,
                                                                        const int *__restrict__ count_p, int slice_bytes,
-                                                                       Operands op, int *__restrict__ rnnz, int tshift)
+                                                                       Operands op, int *__restrict__ rnnz, int tshift,
+                                                                       unsigned *queue)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned redu[16];
+    __shared__ int grp_s;
     const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
     unsigned char *slice = lds + (size_t)g * slice_bytes;
     const int nrows = *count_p;
-    for (int r0 = xcd_block(blockIdx.x, gridDim.x) * RPB; r0 < nrows; r0 += gridDim.x * RPB) {
-        const int idx = r0 + g;
+    const int ngroups = (nrows + RPB - 1) / RPB;
+    int xq = blockIdx.x & 7;
+    for (int grp = queue ? pull_group(queue, ngroups, xq, &grp_s) : xcd_block(blockIdx.x, gridDim.x); grp < ngroups;
+         grp = queue ? pull_group(queue, ngroups, xq, &grp_s) : grp + (int)gridDim.x) {
+        const int idx = grp * RPB + g;
         const int row = idx < nrows ? rows[idx] : -1;
         int S = 0, mn = 0, a0 = 0, nA = 0;
         if (row >= 0) {
@@ -306,17 +311,21 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
                                                                       const int *__restrict__ count_p, int slice_bytes,
                                                                       Operands op, const long long *__restrict__ crp,
                                                                       int *__restrict__ Ccol, typename P::V *__restrict__ Cval,
-                                                                      int bshift, int tshift, int *err)
+                                                                      int bshift, int tshift, int *err, unsigned *queue)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned redu[16];
+    __shared__ int grp_s;
     const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
     unsigned char *slice = lds + (size_t)g * slice_bytes;
     const int nrows = *count_p;
-    for (int r0 = xcd_block(blockIdx.x, gridDim.x) * RPB; r0 < nrows; r0 += gridDim.x * RPB) {
-        const int idx = r0 + g;
+    const int ngroups = (nrows + RPB - 1) / RPB;
+    int xq = blockIdx.x & 7;
+    for (int grp = queue ? pull_group(queue, ngroups, xq, &grp_s) : xcd_block(blockIdx.x, gridDim.x); grp < ngroups;
+         grp = queue ? pull_group(queue, ngroups, xq, &grp_s) : grp + (int)gridDim.x) {
+        const int idx = grp * RPB + g;
         const int row = idx < nrows ? rows[idx] : -1;
         int S = 0, mn = 0, n = 0, a0 = 0, nA = 0;
         long long cb = 0;
@@ -517,36 +526,55 @@ __global__ __launch_bounds__(1024) void k_numeric_global(const int *__restrict__
 
 // ---- launchers -------------------------------------------------------------------------------
 
+// Grid of an LDS row kernel: one workgroup per row group, or, with a work queue, as many
+// workgroups as can be resident at once (occupancy for this LDS size x CUs).
+template <typename Kern>
+static int64_t row_grid(Kern kernel, int bs, size_t lds, int64_t groups, int num_cus, const unsigned *queue)
+{
+    int64_t blocks = groups;
+    if (queue) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bs, lds) != hipSuccess || per_cu < 1) {
+            (void)hipGetLastError();
+            per_cu = 1;
+        }
+        const int64_t resident = (int64_t)per_cu * (num_cus > 0 ? num_cus : 1);
+        blocks = blocks < resident ? blocks : resident;
+    }
+    if (blocks > (1 << 20))
+        blocks = 1 << 20;
+    return blocks < 1 ? 1 : blocks;
+}
+
 template <int G>
 static void symbolic_g(const int *rows, const int *count_dev, int count, int max_need, const Operands &op, int *rnnz,
-                       int tshift, hipStream_t s)
+                       int tshift, int num_cus, unsigned *queue, hipStream_t s)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
     const int slice = (max_need + 15) & ~15;
-    int64_t blocks = ((int64_t)count + RPB - 1) / RPB;
-    if (blocks > (1 << 20))
-        blocks = 1 << 20;
+    const size_t lds = (size_t)slice * RPB;
+    const int64_t blocks = row_grid(k_symbolic_lds<G>, BS, lds, ((int64_t)count + RPB - 1) / RPB, num_cus, queue);
     const int lg = __builtin_ctz(G);
-    hipLaunchKernelGGL((k_symbolic_lds<G>), dim3((unsigned)blocks), dim3(BS), (size_t)slice * RPB, s, rows, count_dev,
-                       slice, op, rnnz, tshift > lg ? lg : tshift);
+    hipLaunchKernelGGL((k_symbolic_lds<G>), dim3((unsigned)blocks), dim3(BS), lds, s, rows, count_dev, slice, op, rnnz,
+                       tshift > lg ? lg : tshift, queue);
 }
 
 hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
                            int *rnnz, int tshift, u64 *gws, long long gws_stride, int num_cus, int gws_blocks,
-                           hipStream_t s)
+                           unsigned *queue, hipStream_t s)
 {
     if (count <= 0)
         return hipSuccess;
     switch (bin) {
-    case 0: symbolic_g<8>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
-    case 1: symbolic_g<16>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
-    case 2: symbolic_g<32>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
-    case 3: symbolic_g<64>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
-    case 4: symbolic_g<128>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
-    case 5: symbolic_g<256>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
-    case 6: symbolic_g<512>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
-    case 7: symbolic_g<1024>(rows, count_dev, count, max_need, op, rnnz, tshift, s); break;
+    case 0: symbolic_g<8>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
+    case 1: symbolic_g<16>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
+    case 2: symbolic_g<32>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
+    case 3: symbolic_g<64>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
+    case 4: symbolic_g<128>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
+    case 5: symbolic_g<256>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
+    case 6: symbolic_g<512>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
+    case 7: symbolic_g<1024>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
     case BIN_GLOBAL: {
         const int blocks = count < gws_blocks ? count : gws_blocks;
         hipLaunchKernelGGL(k_symbolic_global, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, rnnz,
@@ -555,40 +583,39 @@ hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int c
     }
     default: return hipErrorInvalidValue;
     }
-    (void)num_cus;
     return hipGetLastError();
 }
 
 template <int G, typename P>
 static void numeric_g(const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
-                      const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int *err, hipStream_t s)
+                      const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int *err, int num_cus,
+                      unsigned *queue, hipStream_t s)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
     const int slice = (max_need + 15) & ~15;
-    int64_t blocks = ((int64_t)count + RPB - 1) / RPB;
-    if (blocks > (1 << 20))
-        blocks = 1 << 20;
+    const size_t lds = (size_t)slice * RPB;
+    const int64_t blocks = row_grid(k_numeric_lds<G, P>, BS, lds, ((int64_t)count + RPB - 1) / RPB, num_cus, queue);
     const int lg = __builtin_ctz(G);
-    hipLaunchKernelGGL((k_numeric_lds<G, P>), dim3((unsigned)blocks), dim3(BS), (size_t)slice * RPB, s, rows, count_dev,
-                       slice, op, crp, Ccol, (typename P::V *)Cval, bshift > lg ? lg : bshift, tshift > lg ? lg : tshift,
-                       err);
+    hipLaunchKernelGGL((k_numeric_lds<G, P>), dim3((unsigned)blocks), dim3(BS), lds, s, rows, count_dev, slice, op, crp,
+                       Ccol, (typename P::V *)Cval, bshift > lg ? lg : bshift, tshift > lg ? lg : tshift, err, queue);
 }
 
 template <typename P>
 static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
                             const long long *crp, int *Ccol, void *Cval, float *comp, int bshift, int tshift, u64 *gws,
-                            long long gws_stride, long long gws_tiles, int gws_blocks, int *err, hipStream_t s)
+                            long long gws_stride, long long gws_tiles, int gws_blocks, int *err, int num_cus,
+                            unsigned *queue, hipStream_t s)
 {
     switch (bin) {
-    case 0: numeric_g<8, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 1: numeric_g<16, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 2: numeric_g<32, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 3: numeric_g<64, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 4: numeric_g<128, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 5: numeric_g<256, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 6: numeric_g<512, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
-    case 7: numeric_g<1024, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, s); break;
+    case 0: numeric_g<8, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
+    case 1: numeric_g<16, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
+    case 2: numeric_g<32, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
+    case 3: numeric_g<64, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
+    case 4: numeric_g<128, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
+    case 5: numeric_g<256, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
+    case 6: numeric_g<512, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
+    case 7: numeric_g<1024, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
     case BIN_GLOBAL: {
         const int blocks = count < gws_blocks ? count : gws_blocks;
         hipLaunchKernelGGL(k_numeric_global<P>, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, crp, Ccol,
@@ -604,15 +631,15 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
 hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count, int max_need,
                           const Operands &op, const long long *crp, int *Ccol, void *Cval, float *comp, int bshift,
                           int tshift, u64 *gws, long long gws_stride, long long gws_tiles, int gws_blocks, int *err,
-                          hipStream_t s)
+                          int num_cus, unsigned *queue, hipStream_t s)
 {
     if (count <= 0)
         return hipSuccess;
     if (fp32c)
         return numeric_p<PrecF32C>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, comp, bshift, tshift,
-                                   gws, gws_stride, gws_tiles, gws_blocks, err, s);
+                                   gws, gws_stride, gws_tiles, gws_blocks, err, num_cus, queue, s);
     return numeric_p<PrecF64>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, nullptr, bshift, tshift, gws,
-                              gws_stride, gws_tiles, gws_blocks, err, s);
+                              gws_stride, gws_tiles, gws_blocks, err, num_cus, queue, s);
 }
 
 hipError_t configure_kernels(int lds_max)

@@ -178,6 +178,7 @@ const char *hsmu_status_string(int status)
     case HSMU_ERR_NOMEM: return "out of memory";
     case HSMU_ERR_HIP: return "HIP runtime error";
     case HSMU_ERR_UNSUPPORTED: return "unsupported";
+    case HSMU_ERR_PLAN: return "inputs changed structure since the plan was made";
     default: return "unknown status";
     }
 }
@@ -413,28 +414,6 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return t;
 }
 
-// C with M rows and no entries.
-int empty_product(hsmu_ctx *c, int64_t M, int64_t N, hsmu_csr *C, bool rp64, bool plain, bool val32, hsmu_stats *st)
-{
-    const size_t rpb = (size_t)(M + 1) * (rp64 ? 8 : 4);
-    void *rp = out_alloc(c, rpb, plain);
-    if (!rp)
-        return fail(HSMU_ERR_NOMEM, "allocating C row pointers");
-    HIP_TRY(launch_fill_zero(rp, rpb, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    C->M = M;
-    C->N = N;
-    C->nnz = 0;
-    C->rowptr = rp;
-    C->col = nullptr;
-    C->val = nullptr;
-    C->rowptr64 = rp64;
-    C->val32 = val32;
-    if (st)
-        memset(st, 0, sizeof(*st));
-    return HSMU_OK;
-}
-
 } // namespace
 
 extern "C" int hsmu_btiles(hsmu_ctx *c, const hsmu_csr *B, hsmu_tiles *T)
@@ -506,72 +485,154 @@ extern "C" int hsmu_csr_free(hsmu_ctx *c, hsmu_csr *C)
     return HSMU_OK;
 }
 
-extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C, const hsmu_opts *opts_in,
-                           hsmu_stats *stats)
-{
-    if (!c || !C)
-        return fail(HSMU_ERR_INVALID, "hsmu_spgemm: ctx or C is NULL");
+// A plan: the recorded run of one batch (or one SpGEMM) and its captured HIP graph.
+struct hsmu_plan {
+    hsmu_ctx *c = nullptr;
     hsmu_opts opts;
-    if (opts_in)
-        opts = *opts_in;
-    else
-        hsmu_opts_default(&opts);
-    if (opts.precision != HSMU_FP64 && opts.precision != HSMU_FP32_COMP)
-        return fail(HSMU_ERR_INVALID, "unknown precision %d", opts.precision);
+    std::vector<void *> keep;     // every device block the plan uses, in allocation order
+    DevStats rec[2];              // the two checkpoints of the recording run
+    DevStats *expect = nullptr;   // device copy of rec, compared by the replay
+    DevStats *dst = nullptr;      // device counters of the plan's runs
+    hsmu_stats stats0;            // statistics of the recording run
+    bool empty = false;           // product with no entries: nothing to replay
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// Device memory of one pipeline run.  A one-shot call takes its workspace from the pool
+// and gives it back on exit; its outputs survive the call.  A plan keeps every block,
+// workspace and outputs alike, until it is destroyed, and its capture pass is handed the
+// same blocks again in the same order (the sequence of requests depends only on the
+// recorded checkpoints, which the capture pass reuses).
+class Arena {
+  public:
+    Arena(hsmu_ctx *c, std::vector<void *> *keep, bool replay) : c_(c), keep_(keep), replay_(replay) {}
+    ~Arena()
+    {
+        for (void *p : taken_)
+            c_->pool.release(p);
+    }
+    void *raw(size_t bytes)
+    {
+        if (replay_)
+            return pos_ < keep_->size() ? (*keep_)[pos_++] : nullptr;
+        void *p = c_->pool.alloc(bytes);
+        if (p)
+            (keep_ ? *keep_ : taken_).push_back(p);
+        return p;
+    }
+    template <typename T> T *get(size_t n) { return (T *)raw(n * sizeof(T)); }
+    // an output array of a one-shot call (plain hipMalloc or the pool, not given back at
+    // exit); a plan's outputs are kept blocks like the rest
+    void *out(size_t bytes, bool plain) { return keep_ ? raw(bytes) : out_alloc(c_, bytes, plain); }
+    void drop_out(void *p)
+    {
+        if (!keep_)
+            out_free(c_, p);
+    }
+
+  private:
+    hsmu_ctx *c_;
+    std::vector<void *> *keep_;
+    bool replay_;
+    size_t pos_ = 0;
+    std::vector<void *> taken_;
+};
+
+// How a pipeline run meets its two checkpoints: eagerly (copy the device counters to
+// the host and synchronise; a plan's recording run keeps them), or, while a plan is
+// being captured, with no synchronisation at all: the recorded counters stand in for the
+// device's, and a check kernel compares the two when the graph is replayed.
+struct RunMode {
+    hsmu_plan *plan = nullptr;
+    bool capture = false;
+};
+
+int checkpoint(hsmu_ctx *c, const RunMode &m, DevStats *dst, int k, DevStats &out)
+{
+    hipStream_t s = c->stream;
+    if (m.capture) {
+        HIP_TRY(launch_plan_check(dst, m.plan->expect + k, k, s));
+        out = m.plan->rec[k];
+        return HSMU_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(c->host_stats, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    out = *c->host_stats;
+    if (m.plan)
+        m.plan->rec[k] = out;
+    return device_error_status(out.err);
+}
+
+// The operands of one pipeline run (device arrays, int32 row pointers): one SpGEMM, or
+// the stacked instances of a batch.
+struct Problem {
+    int64_t M, K, N, nnzA, nnzB;
+    const int *Arp, *Acol;
+    const void *Aval;
+    const int *Brp, *Bcol;
+    const void *Bval;
+};
+
+// Set for a batch: the output row pointers are split per instance instead of converted.
+struct BatchSplit {
+    int64_t count;
+    BatchOffsets off; // device
+    std::vector<long long> *bases; // host: where each C_i's entries start (eager runs)
+    long long *bases_dev = nullptr; // the same on the device (set by the pipeline)
+};
+
+struct Output {
+    int64_t nnzc = 0;
+    void *rowptr = nullptr;
+    int *col = nullptr;
+    void *val = nullptr;
+};
+
+// B -> tiles, row analysis, symbolic, C allocation, numeric.  *dst must be zeroed on the
+// stream before the call (the batch stacking kernels report into it first).
+int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, const RunMode &mode, DevStats *dst,
+             BatchSplit *batch, Output &res, hsmu_stats *stats)
+{
     const bool fp32 = opts.precision == HSMU_FP32_COMP;
     const double vsz = fp32 ? 4.0 : 8.0; // bytes per value
-    int rc = check_input(A, "A", fp32);
-    if (!rc)
-        rc = check_input(B, "B", fp32);
-    if (rc)
-        return rc;
-    if (A->N != B->M)
-        return fail(HSMU_ERR_INVALID, "A is %lld x %lld but B has %lld rows", (long long)A->M, (long long)A->N,
-                    (long long)B->M);
-    HIP_TRY(hipSetDevice(c->device));
-    memset(C, 0, sizeof(*C));
+    const bool cap = mode.capture;
     const bool rp64 = opts.rowptr64 != 0;
     const bool plain = opts.plain_alloc != 0;
-    const int64_t M = A->M, K = B->M, N = B->N;
-    if (stats)
-        memset(stats, 0, sizeof(*stats));
-    if (M == 0 || A->nnz == 0 || B->nnz == 0)
-        return empty_product(c, M, N, C, rp64, plain, fp32, stats);
-
+    const int64_t M = P.M, K = P.K, N = P.N;
     hipStream_t s = c->stream;
-    const bool timing = opts.timing != 0;
-    Scratch ws(c->pool);
-    DevStats *dst = ws.get<DevStats>(1);
-    int *tcnt = (int *)ws.get<char>(btiles_ws_bytes(B->nnz));
-    int *tptr = ws.get<int>((size_t)K + 1);
-    int *tcol = ws.get<int>((size_t)B->nnz);
-    unsigned long long *tmask = ws.get<unsigned long long>((size_t)B->nnz);
+    const bool timing = opts.timing != 0 && !cap;
+    auto mark = [&](hipEvent_t e) -> hipError_t { return cap ? hipSuccess : hipEventRecord(e, s); };
+    int rc;
+
+    int *tcnt = (int *)ar.get<char>(btiles_ws_bytes(P.nnzB));
+    int *tptr = ar.get<int>((size_t)K + 1);
+    int *tcol = ar.get<int>((size_t)P.nnzB);
+    unsigned long long *tmask = ar.get<unsigned long long>((size_t)P.nnzB);
     int64_t nmax = M > K ? M : K;
-    nmax = nmax > B->nnz ? nmax : B->nnz;
-    void *scan_ws = ws.get<char>(scan_ws_bytes(nmax + 1));
-    long long *rprod = ws.get<long long>((size_t)M);
-    int *rtp = ws.get<int>((size_t)M);
-    int *rmin = ws.get<int>((size_t)M);
-    int *rmax = ws.get<int>((size_t)M);
-    int *rnnz = ws.get<int>((size_t)M);
-    int *binrows = ws.get<int>((size_t)NBINS * (size_t)M);
-    long long *rdir = ws.get<long long>((size_t)M);
-    if (!dst || !tcnt || !tptr || !tcol || !tmask || !scan_ws || !rprod || !rtp || !rmin || !rmax || !rnnz || !binrows ||
-        !rdir)
+    nmax = nmax > P.nnzB ? nmax : P.nnzB;
+    void *scan_ws = ar.get<char>(scan_ws_bytes(nmax + 1));
+    long long *rprod = ar.get<long long>((size_t)M);
+    int *rtp = ar.get<int>((size_t)M);
+    int *rmin = ar.get<int>((size_t)M);
+    int *rmax = ar.get<int>((size_t)M);
+    int *rnnz = ar.get<int>((size_t)M);
+    int *binrows = ar.get<int>((size_t)NBINS * (size_t)M);
+    long long *rdir = ar.get<long long>((size_t)M);
+    if (!tcnt || !tptr || !tcol || !tmask || !scan_ws || !rprod || !rtp || !rmin || !rmax || !rnnz || !binrows || !rdir)
         return fail(HSMU_ERR_NOMEM, "workspace allocation");
 
     // ---- 1. B -> tiles ----
-    HIP_TRY(hipEventRecord(c->ev[EV_START], s));
-    HIP_TRY(launch_fill_zero(dst, sizeof(DevStats), s));
-    HIP_TRY(launch_btiles(K, N, B->nnz, (const int *)B->rowptr, B->col, tcnt, tptr, tcol, tmask, scan_ws, dst,
-                          c->num_cus, s));
-    HIP_TRY(hipEventRecord(c->ev[EV_CONV], s));
+    HIP_TRY(launch_btiles(K, N, P.nnzB, P.Brp, P.Bcol, tcnt, tptr, tcol, tmask, scan_ws, dst, c->num_cus, s));
+    HIP_TRY(mark(c->ev[EV_CONV]));
 
     // ---- 2. row analysis + symbolic binning ----
-    const int avg_a = (int)(A->nnz / (M > 0 ? M : 1));
+    const int avg_a = (int)(P.nnzA / (M > 0 ? M : 1));
     // the spare bin region (bin 9) holds the list of long rows until binning runs
-    HIP_TRY(launch_row_info(M, K, A->nnz, (const int *)A->rowptr, A->col, (const int *)B->rowptr, tptr, tcol, rprod,
-                            rtp, rmin, rmax, rnnz, binrows + (size_t)(NBINS - 1) * M, dst, avg_a, c->num_cus, s));
+    HIP_TRY(launch_row_info(M, K, P.nnzA, P.Arp, P.Acol, P.Brp, tptr, tcol, rprod, rtp, rmin, rmax, rnnz,
+                            binrows + (size_t)(NBINS - 1) * M, dst, avg_a, c->num_cus, s));
     BinParams bp;
     const int lds_dyn = c->lds_max - LDS_STATIC_RESERVE;
     bp.lds_row_max = opts.lds_row_max > 0 ? opts.lds_row_max : lds_dyn;
@@ -583,17 +644,15 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     bp.per_lane = opts.tiles_per_lane > 0 ? opts.tiles_per_lane : 4;
     bp.force_global = opts.force_global;
     bp.pad = 0;
-    HIP_TRY(launch_bin(0, M, (const int *)A->rowptr, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, s));
-    HIP_TRY(hipEventRecord(c->ev[EV_ANALYSIS], s));
-    HIP_TRY(hipMemcpyAsync(c->host_stats, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    const DevStats h1 = *c->host_stats;
-    rc = device_error_status(h1.err);
+    HIP_TRY(launch_bin(0, M, P.Arp, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, s));
+    HIP_TRY(mark(c->ev[EV_ANALYSIS]));
+    DevStats h1;
+    rc = checkpoint(c, mode, dst, 0, h1);
     if (rc)
         return rc;
 
     // lanes per A entry: walk one B row (numeric) / one B tile row (tile OR)
-    const long long avg_b = B->nnz / (K > 0 ? K : 1);
+    const long long avg_b = P.nnzB / (K > 0 ? K : 1);
     const long long avg_t = (long long)h1.ntiles_b / (K > 0 ? K : 1);
     // about four products per lane per B row: shorter sub-groups keep more A entries in
     // flight per wave and idle fewer lanes on ragged B rows (measured on the banded
@@ -621,17 +680,17 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
         if (blocks < 1)
             blocks = 1;
         gws_blocks = blocks;
-        gws = ws.get<unsigned long long>((size_t)gws_stride * (size_t)blocks);
+        gws = ar.get<unsigned long long>((size_t)gws_stride * (size_t)blocks);
         return gws ? HSMU_OK : fail(HSMU_ERR_NOMEM, "global accumulator workspace");
     };
 
     Operands op;
-    op.Arp = (const int *)A->rowptr;
-    op.Acol = A->col;
-    op.Aval = A->val;
-    op.Brp = (const int *)B->rowptr;
-    op.Bcol = B->col;
-    op.Bval = B->val;
+    op.Arp = P.Arp;
+    op.Acol = P.Acol;
+    op.Aval = P.Aval;
+    op.Brp = P.Brp;
+    op.Bcol = P.Bcol;
+    op.Bval = P.Bval;
     op.val32 = fp32 ? 1 : 0;
     op.tptr = tptr;
     op.tcol = tcol;
@@ -647,8 +706,8 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     const unsigned long long dir_total = h1.dir_total;
     const unsigned long long dir_budget = 1ull << 31;
     if (!opts.rebuild_dirs && dir_total > 0 && dir_total * 12ull <= dir_budget) {
-        op.dir_mask = ws.get<unsigned long long>((size_t)dir_total);
-        op.dir_off = ws.get<unsigned>((size_t)dir_total);
+        op.dir_mask = ar.get<unsigned long long>((size_t)dir_total);
+        op.dir_off = ar.get<unsigned>((size_t)dir_total);
         if (!op.dir_mask || !op.dir_off) {
             op.dir_mask = nullptr;
             op.dir_off = nullptr;
@@ -666,29 +725,36 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
                 return rc;
         }
         HIP_TRY(launch_symbolic(b, binrows + (size_t)b * M, &dst->sym.count[b], cnt, h1.sym.max_need[b], op, rnnz,
-                                tshift, gws, gws_stride, c->num_cus, gws_blocks, s));
+                                tshift, gws, gws_stride, c->num_cus, gws_blocks,
+                                opts.persistent ? dst->queue[0][b] : nullptr, s));
     }
-    // C row pointers (int64); they become C's rowptr when int64 output is asked for
-    long long *crp = (long long *)out_alloc(c, ((size_t)M + 1) * sizeof(long long), plain && rp64);
+    // Arrays of C handed to the caller; dropped again (one-shot calls) on any failure.
+    struct Outs {
+        Arena &ar;
+        std::vector<void *> p;
+        ~Outs()
+        {
+            for (void *q : p)
+                ar.drop_out(q);
+        }
+    } outs{ar, {}};
+    // C row pointers (int64); they are C's rowptr when a single product asks for int64
+    long long *crp;
+    if (rp64 && !batch) {
+        crp = (long long *)ar.out(((size_t)M + 1) * sizeof(long long), plain);
+        if (crp)
+            outs.p.push_back(crp);
+    } else {
+        crp = ar.get<long long>((size_t)M + 1);
+    }
     if (!crp)
         return fail(HSMU_ERR_NOMEM, "C row pointers");
-    struct CrpGuard {
-        hsmu_ctx *c;
-        void *p;
-        ~CrpGuard()
-        {
-            if (p)
-                out_free(c, p);
-        }
-    } crp_guard{c, crp};
     HIP_TRY(launch_scan_i32_i64(rnnz, M, crp, scan_ws, &dst->nnz_c, s));
     bp.per_lane = opts.products_per_lane > 0 ? opts.products_per_lane : 32;
-    HIP_TRY(launch_bin(1, M, (const int *)A->rowptr, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, s));
-    HIP_TRY(hipEventRecord(c->ev[EV_SYMBOLIC], s));
-    HIP_TRY(hipMemcpyAsync(c->host_stats, dst, sizeof(DevStats), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    const DevStats h2 = *c->host_stats;
-    rc = device_error_status(h2.err);
+    HIP_TRY(launch_bin(1, M, P.Arp, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, s));
+    HIP_TRY(mark(c->ev[EV_SYMBOLIC]));
+    DevStats h2;
+    rc = checkpoint(c, mode, dst, 1, h2);
     if (rc)
         return rc;
     const long long nnzc = h2.nnz_c;
@@ -697,19 +763,28 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
 
     // ---- 4. allocate C ----
     const auto t_alloc0 = std::chrono::steady_clock::now();
-    int *Ccol = (int *)out_alloc(c, (size_t)nnzc * sizeof(int), plain);
-    void *Cval = out_alloc(c, (size_t)nnzc * (size_t)vsz, plain);
-    void *Crp = rp64 ? (void *)crp : out_alloc(c, ((size_t)M + 1) * sizeof(int), plain);
-    if (!Ccol || !Cval || !Crp) {
-        out_free(c, Ccol);
-        out_free(c, Cval);
-        if (!rp64)
-            out_free(c, Crp);
-        return fail(HSMU_ERR_NOMEM, "allocating C (nnz %lld)", nnzc);
+    int *Ccol = (int *)ar.out((size_t)nnzc * sizeof(int), plain);
+    outs.p.push_back(Ccol);
+    void *Cval = ar.out((size_t)nnzc * (size_t)vsz, plain);
+    outs.p.push_back(Cval);
+    void *Crp;
+    long long *bases = nullptr;
+    if (batch) {
+        Crp = ar.out((size_t)(M + batch->count) * (rp64 ? 8 : 4), plain);
+        outs.p.push_back(Crp);
+        bases = ar.get<long long>((size_t)batch->count + 1);
+        batch->bases_dev = bases;
+    } else if (rp64) {
+        Crp = crp;
+    } else {
+        Crp = ar.out(((size_t)M + 1) * sizeof(int), plain);
+        outs.p.push_back(Crp);
     }
+    if (!Ccol || !Cval || !Crp || (batch && !bases))
+        return fail(HSMU_ERR_NOMEM, "allocating C (nnz %lld)", nnzc);
     const double ms_alloc_host =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_alloc0).count();
-    HIP_TRY(hipEventRecord(c->ev[EV_ALLOC], s));
+    HIP_TRY(mark(c->ev[EV_ALLOC]));
 
     // ---- 5. numeric ----
     int *errp = &dst->err;
@@ -725,41 +800,43 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
         // compensation terms of FP32C rows accumulated in global memory
         float *comp = nullptr;
         if (fp32 && b == BIN_GLOBAL) {
-            comp = ws.get<float>((size_t)nnzc);
+            comp = ar.get<float>((size_t)nnzc);
             if (!comp)
                 return fail(HSMU_ERR_NOMEM, "compensation scratch (nnz %lld)", nnzc);
         }
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][0], s));
         HIP_TRY(launch_numeric(b, fp32 ? 1 : 0, binrows + (size_t)b * M, &dst->num.count[b], cnt, h2.num.max_need[b], op,
-                               crp, Ccol, Cval, comp, bshift, tshift, gws, gws_stride, gws_tiles, gws_blocks, errp, s));
+                               crp, Ccol, Cval, comp, bshift, tshift, gws, gws_stride, gws_tiles, gws_blocks, errp,
+                               c->num_cus, opts.persistent ? dst->queue[1][b] : nullptr, s));
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][1], s));
     }
-    if (!rp64)
+    if (batch)
+        HIP_TRY(launch_batch_split(batch->count, batch->off, crp, Crp, rp64 ? 1 : 0, bases, M + batch->count, s));
+    else if (!rp64)
         HIP_TRY(launch_rowptr_i64_to_i32(crp, M + 1, (int *)Crp, s));
-    HIP_TRY(hipEventRecord(c->ev[EV_NUMERIC], s));
-    HIP_TRY(hipMemcpyAsync(&c->host_stats->err, errp, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    rc = device_error_status(c->host_stats->err);
-    if (rc) {
-        out_free(c, Ccol);
-        out_free(c, Cval);
-        if (!rp64)
-            out_free(c, Crp);
-        return rc;
+    HIP_TRY(mark(c->ev[EV_NUMERIC]));
+    if (!cap) {
+        HIP_TRY(hipMemcpyAsync(&c->host_stats->err, errp, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (batch) {
+            batch->bases->assign((size_t)batch->count + 1, 0);
+            HIP_TRY(hipMemcpyAsync(batch->bases->data(), bases, ((size_t)batch->count + 1) * sizeof(long long),
+                                   hipMemcpyDeviceToHost, s));
+        }
+        HIP_TRY(hipStreamSynchronize(s));
+        rc = device_error_status(c->host_stats->err);
+        if (rc)
+            return rc;
     }
-    if (rp64)
-        crp_guard.p = nullptr; // crp is now C's row pointer array
+    outs.p.clear(); // the arrays now belong to the result
 
-    C->M = M;
-    C->N = N;
-    C->nnz = nnzc;
-    C->rowptr = Crp;
-    C->col = Ccol;
-    C->val = Cval;
-    C->rowptr64 = rp64;
-    C->val32 = fp32 ? 1 : 0;
+    res.nnzc = nnzc;
+    res.rowptr = Crp;
+    res.col = Ccol;
+    res.val = Cval;
+    if (cap)
+        return HSMU_OK;
 
     if (stats) {
         hsmu_stats &S = *stats;
@@ -799,8 +876,457 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
         S.gflops = S.ms_time2 > 0 ? 2.0 * (double)S.products / (S.ms_time2 * 1e6) : 0.0;
         S.space_compressed_mask_mb =
             ((double)(K + 1) * 4.0 + (double)h2.ntiles_b * (4.0 + 8.0)) / (1024.0 * 1024.0);
-        S.alg_bytes = 4.0 * (double)(M + 1) + (12.0 + vsz) * (double)A->nnz + (4.0 + vsz) * (double)S.products +
+        S.alg_bytes = 4.0 * (double)(M + 1) + (12.0 + vsz) * (double)P.nnzA + (4.0 + vsz) * (double)S.products +
                       (rp64 ? 8.0 : 4.0) * (double)(M + 1) + (4.0 + vsz) * (double)nnzc;
+    }
+    return HSMU_OK;
+}
+
+hsmu_opts resolve_opts(const hsmu_opts *in)
+{
+    hsmu_opts o;
+    if (in)
+        o = *in;
+    else
+        hsmu_opts_default(&o);
+    return o;
+}
+
+int check_opts(const hsmu_opts &o)
+{
+    if (o.precision != HSMU_FP64 && o.precision != HSMU_FP32_COMP)
+        return fail(HSMU_ERR_INVALID, "unknown precision %d", o.precision);
+    return HSMU_OK;
+}
+
+int check_pair(const hsmu_csr *A, const hsmu_csr *B, bool fp32)
+{
+    int rc = check_input(A, "A", fp32);
+    if (!rc)
+        rc = check_input(B, "B", fp32);
+    if (rc)
+        return rc;
+    if (A->N != B->M)
+        return fail(HSMU_ERR_INVALID, "A is %lld x %lld but B has %lld rows", (long long)A->M, (long long)A->N,
+                    (long long)B->M);
+    return HSMU_OK;
+}
+
+Problem single_problem(const hsmu_csr *A, const hsmu_csr *B)
+{
+    Problem P;
+    P.M = A->M;
+    P.K = B->M;
+    P.N = B->N;
+    P.nnzA = A->nnz;
+    P.nnzB = B->nnz;
+    P.Arp = (const int *)A->rowptr;
+    P.Acol = A->col;
+    P.Aval = A->val;
+    P.Brp = (const int *)B->rowptr;
+    P.Bcol = B->col;
+    P.Bval = B->val;
+    return P;
+}
+
+void fill_result(hsmu_csr *C, const Output &r, int64_t M, int64_t N, const hsmu_opts &o)
+{
+    C->M = M;
+    C->N = N;
+    C->nnz = r.nnzc;
+    C->rowptr = r.rowptr;
+    C->col = r.col;
+    C->val = r.val;
+    C->rowptr64 = o.rowptr64 != 0;
+    C->val32 = o.precision == HSMU_FP32_COMP;
+}
+
+// ---- batches -------------------------------------------------------------------------------
+
+// Host side of a batch: the instance table, the offsets and the stacked sizes.
+struct BatchPrep {
+    int64_t count = 0;
+    std::vector<BatchInst> inst;
+    std::vector<long long> offs; // mo, ko, ea, eb: count + 1 each
+    bool self = false;           // every instance is A_i * A_i on the same arrays
+    int64_t M = 0, K = 0, N = 0, nnzA = 0, nnzB = 0;
+};
+
+int prepare_batch(int64_t count, const hsmu_csr *A, const hsmu_csr *B, bool fp32, BatchPrep &bp)
+{
+    if (count < 1)
+        return fail(HSMU_ERR_INVALID, "batch count %lld", (long long)count);
+    if (count >= INT32_MAX)
+        return fail(HSMU_ERR_UNSUPPORTED, "batch count %lld", (long long)count);
+    if (!A || !B)
+        return fail(HSMU_ERR_INVALID, "batch A or B is NULL");
+    bp.count = count;
+    bp.inst.resize((size_t)count);
+    bp.offs.assign(4 * ((size_t)count + 1), 0);
+    long long *mo = bp.offs.data(), *ko = mo + count + 1, *ea = ko + count + 1, *eb = ea + count + 1;
+    bool self = true;
+    for (int64_t i = 0; i < count; ++i) {
+        int rc = check_pair(A + i, B + i, fp32);
+        if (rc)
+            return fail(rc, "instance %lld: %s", (long long)i, hsmu_last_error());
+        if ((A[i].M == 0 && A[i].nnz > 0) || (B[i].M == 0 && B[i].nnz > 0))
+            return fail(HSMU_ERR_INVALID, "instance %lld: entries in a matrix without rows", (long long)i);
+        BatchInst &I = bp.inst[(size_t)i];
+        I.Arp = (const int *)A[i].rowptr;
+        I.Acol = A[i].col;
+        I.Aval = A[i].val;
+        I.Brp = (const int *)B[i].rowptr;
+        I.Bcol = B[i].col;
+        I.Bval = B[i].val;
+        I.M = (int)A[i].M;
+        I.K = (int)B[i].M;
+        I.N = (int)B[i].N;
+        I.pad = 0;
+        mo[i + 1] = mo[i] + A[i].M;
+        ko[i + 1] = ko[i] + B[i].M;
+        ea[i + 1] = ea[i] + A[i].nnz;
+        eb[i + 1] = eb[i] + B[i].nnz;
+        bp.N = B[i].N > bp.N ? B[i].N : bp.N;
+        self = self && A[i].rowptr == B[i].rowptr && A[i].col == B[i].col && A[i].val == B[i].val &&
+               A[i].M == B[i].M && A[i].nnz == B[i].nnz;
+    }
+    bp.M = mo[count];
+    bp.K = ko[count];
+    bp.nnzA = ea[count];
+    bp.nnzB = eb[count];
+    bp.self = self;
+    if (bp.M + count >= INT32_MAX || bp.K >= INT32_MAX || bp.nnzA > INT32_MAX || bp.nnzB > INT32_MAX)
+        return fail(HSMU_ERR_UNSUPPORTED, "stacked batch beyond int32 indices (rows %lld, nnz %lld / %lld)",
+                    (long long)bp.M, (long long)bp.nnzA, (long long)bp.nnzB);
+    return HSMU_OK;
+}
+
+// Stack the instances on the device (the table and offsets are uploaded by eager runs
+// only: a capture pass reuses what the recording run uploaded into the same blocks).
+int stack_inputs(hsmu_ctx *c, const BatchPrep &bp, const hsmu_opts &opts, Arena &ar, const RunMode &mode,
+                 DevStats *dst, Problem &P, BatchOffsets &off)
+{
+    hipStream_t s = c->stream;
+    const bool fp32 = opts.precision == HSMU_FP32_COMP;
+    const size_t vsz = fp32 ? 4 : 8;
+    const int64_t n1 = bp.count + 1;
+    BatchInst *inst = ar.get<BatchInst>((size_t)bp.count);
+    long long *offs = ar.get<long long>(4 * (size_t)n1);
+    int *Brp = ar.get<int>((size_t)bp.K + 1);
+    int *Bcol = ar.get<int>((size_t)bp.nnzB);
+    void *Bval = ar.raw((size_t)bp.nnzB * vsz);
+    int *Acol = ar.get<int>((size_t)bp.nnzA);
+    int *Arp = bp.self ? Brp : ar.get<int>((size_t)bp.M + 1);
+    void *Aval = bp.self ? Bval : ar.raw((size_t)bp.nnzA * vsz);
+    if (!inst || !offs || !Brp || !Bcol || !Bval || !Acol || !Arp || !Aval)
+        return fail(HSMU_ERR_NOMEM, "stacked batch inputs");
+    if (!mode.capture) {
+        HIP_TRY(hipMemcpyAsync(inst, bp.inst.data(), bp.inst.size() * sizeof(BatchInst), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(offs, bp.offs.data(), bp.offs.size() * sizeof(long long), hipMemcpyHostToDevice, s));
+    }
+    off.mo = offs;
+    off.ko = offs + n1;
+    off.ea = offs + 2 * n1;
+    off.eb = offs + 3 * n1;
+    const int v32 = fp32 ? 1 : 0;
+    HIP_TRY(launch_batch_rowptr(1, bp.count, inst, off, bp.K, bp.nnzB, Brp, dst, s));
+    HIP_TRY(launch_batch_entries(1, bp.count, inst, off, bp.nnzB, v32, Bcol, Bval, dst, s));
+    if (!bp.self)
+        HIP_TRY(launch_batch_rowptr(0, bp.count, inst, off, bp.M, bp.nnzA, Arp, dst, s));
+    HIP_TRY(launch_batch_entries(0, bp.count, inst, off, bp.nnzA, v32, Acol, bp.self ? nullptr : Aval, dst, s));
+    P.M = bp.M;
+    P.K = bp.K;
+    P.N = bp.N;
+    P.nnzA = bp.nnzA;
+    P.nnzB = bp.nnzB;
+    P.Arp = Arp;
+    P.Acol = Acol;
+    P.Aval = Aval;
+    P.Brp = Brp;
+    P.Bcol = Bcol;
+    P.Bval = Bval;
+    return HSMU_OK;
+}
+
+// Views of the batch's result: C_i's row pointers at rowptr + mo_i + i, entries at bases[i].
+void fill_views(const BatchPrep &bp, const hsmu_csr *B, const Output &r, const std::vector<long long> &bases,
+                const hsmu_opts &o, hsmu_csr *C)
+{
+    const bool rp64 = o.rowptr64 != 0;
+    const size_t vsz = o.precision == HSMU_FP32_COMP ? 4 : 8;
+    const long long *mo = bp.offs.data();
+    for (int64_t i = 0; i < bp.count; ++i) {
+        hsmu_csr &X = C[i];
+        X.M = bp.inst[(size_t)i].M;
+        X.N = B[i].N;
+        X.nnz = r.col ? bases[(size_t)i + 1] - bases[(size_t)i] : 0;
+        X.rowptr = (char *)r.rowptr + (size_t)(mo[i] + i) * (rp64 ? 8 : 4);
+        X.col = r.col ? r.col + bases[(size_t)i] : nullptr;
+        X.val = r.col ? (char *)r.val + (size_t)bases[(size_t)i] * vsz : nullptr;
+        X.rowptr64 = rp64;
+        X.val32 = o.precision == HSMU_FP32_COMP;
+    }
+}
+
+// Run a batch (count > 1) or a single product (count == 1) through the pipeline; with a
+// plan, in its recording or capture pass.  Fills C (views for a batch).
+int run_batch(hsmu_ctx *c, int64_t count, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C, const hsmu_opts &opts,
+              Arena &ar, const RunMode &mode, hsmu_stats *stats, bool *empty)
+{
+    const bool fp32 = opts.precision == HSMU_FP32_COMP;
+    hipStream_t s = c->stream;
+    *empty = false;
+    BatchPrep bp;
+    int rc;
+    if (count == 1) {
+        rc = check_pair(A, B, fp32);
+        if (rc)
+            return rc;
+    } else {
+        rc = prepare_batch(count, A, B, fp32, bp);
+        if (rc)
+            return rc;
+    }
+    const int64_t M = count == 1 ? A->M : bp.M;
+    const int64_t nnzA = count == 1 ? A->nnz : bp.nnzA, nnzB = count == 1 ? B->nnz : bp.nnzB;
+    if (M == 0 || nnzA == 0 || nnzB == 0) {
+        // no products: C has row pointers only, all zero
+        *empty = true;
+        if (stats)
+            memset(stats, 0, sizeof(*stats));
+        if (mode.capture)
+            return HSMU_OK;
+        const bool rp64 = opts.rowptr64 != 0;
+        const size_t rpb = (size_t)(M + count) * (rp64 ? 8 : 4);
+        void *rp = ar.out(rpb, opts.plain_alloc != 0);
+        if (!rp)
+            return fail(HSMU_ERR_NOMEM, "allocating C row pointers");
+        HIP_TRY(launch_fill_zero(rp, rpb, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        Output r;
+        r.rowptr = rp;
+        if (count == 1) {
+            fill_result(C, r, A->M, B->N, opts);
+        } else {
+            std::vector<long long> bases((size_t)count + 1, 0);
+            fill_views(bp, B, r, bases, opts, C);
+        }
+        return HSMU_OK;
+    }
+    DevStats *dst = ar.get<DevStats>(1);
+    if (!dst)
+        return fail(HSMU_ERR_NOMEM, "workspace allocation");
+    if (mode.plan && !mode.capture) {
+        mode.plan->dst = dst;
+        mode.plan->expect = ar.get<DevStats>(2);
+        if (!mode.plan->expect)
+            return fail(HSMU_ERR_NOMEM, "workspace allocation");
+    } else if (mode.capture) {
+        ar.get<DevStats>(2); // the recorded expect block, in allocation order
+    }
+    if (!mode.capture)
+        HIP_TRY(hipEventRecord(c->ev[EV_START], s));
+    HIP_TRY(launch_fill_zero(dst, sizeof(DevStats), s));
+    Output r;
+    if (count == 1) {
+        rc = pipeline(c, single_problem(A, B), opts, ar, mode, dst, nullptr, r, stats);
+        if (rc)
+            return rc;
+        if (!mode.capture)
+            fill_result(C, r, A->M, B->N, opts);
+        return HSMU_OK;
+    }
+    Problem P;
+    BatchSplit split;
+    std::vector<long long> bases;
+    split.count = count;
+    split.bases = &bases;
+    rc = stack_inputs(c, bp, opts, ar, mode, dst, P, split.off);
+    if (rc)
+        return rc;
+    rc = pipeline(c, P, opts, ar, mode, dst, &split, r, stats);
+    if (rc)
+        return rc;
+    if (mode.plan) {
+        // a replay must leave every C_i where the views point: compare the starts
+        long long *want = ar.get<long long>((size_t)count + 1);
+        if (!want)
+            return fail(HSMU_ERR_NOMEM, "plan workspace");
+        if (mode.capture) {
+            HIP_TRY(launch_plan_bases(split.bases_dev, want, count + 1, dst, s));
+        } else {
+            HIP_TRY(hipMemcpyAsync(want, bases.data(), bases.size() * sizeof(long long), hipMemcpyHostToDevice, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+    }
+    if (!mode.capture)
+        fill_views(bp, B, r, bases, opts, C);
+    return HSMU_OK;
+}
+
+} // namespace
+
+extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C, const hsmu_opts *opts_in,
+                           hsmu_stats *stats)
+{
+    if (!c || !C)
+        return fail(HSMU_ERR_INVALID, "hsmu_spgemm: ctx or C is NULL");
+    const hsmu_opts opts = resolve_opts(opts_in);
+    int rc = check_opts(opts);
+    if (rc)
+        return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    memset(C, 0, sizeof(*C));
+    if (stats)
+        memset(stats, 0, sizeof(*stats));
+    Arena ar(c, nullptr, false);
+    bool empty;
+    return run_batch(c, 1, A, B, C, opts, ar, RunMode(), stats, &empty);
+}
+
+extern "C" int hsmu_spgemm_batched(hsmu_ctx *c, int64_t count, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C,
+                                   const hsmu_opts *opts_in, hsmu_stats *stats)
+{
+    if (!c || !C || count < 1)
+        return fail(HSMU_ERR_INVALID, "hsmu_spgemm_batched: ctx or C is NULL, or count < 1");
+    const hsmu_opts opts = resolve_opts(opts_in);
+    int rc = check_opts(opts);
+    if (rc)
+        return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    memset(C, 0, (size_t)count * sizeof(*C));
+    if (stats)
+        memset(stats, 0, sizeof(*stats));
+    Arena ar(c, nullptr, false);
+    bool empty;
+    return run_batch(c, count, A, B, C, opts, ar, RunMode(), stats, &empty);
+}
+
+extern "C" int hsmu_batch_free(hsmu_ctx *c, int64_t count, hsmu_csr *C)
+{
+    if (!c || !C || count < 1)
+        return fail(HSMU_ERR_INVALID, "hsmu_batch_free: NULL argument or count < 1");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    out_free(c, C[0].rowptr);
+    out_free(c, C[0].col);
+    out_free(c, C[0].val);
+    memset(C, 0, (size_t)count * sizeof(*C));
+    return HSMU_OK;
+}
+
+// ---- plans -------------------------------------------------------------------------------------
+
+extern "C" void hsmu_plan_destroy(hsmu_plan *p)
+{
+    if (!p)
+        return;
+    hsmu_ctx *c = p->c;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (p->exec)
+        (void)hipGraphExecDestroy(p->exec);
+    if (p->graph)
+        (void)hipGraphDestroy(p->graph);
+    for (void *q : p->keep)
+        c->pool.release(q);
+    delete p;
+}
+
+extern "C" int hsmu_plan_create(hsmu_ctx *c, int64_t count, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C,
+                                const hsmu_opts *opts_in, hsmu_plan **out)
+{
+    if (!c || !C || !out || count < 1)
+        return fail(HSMU_ERR_INVALID, "hsmu_plan_create: NULL argument or count < 1");
+    *out = nullptr;
+    hsmu_opts opts = resolve_opts(opts_in);
+    int rc = check_opts(opts);
+    if (rc)
+        return rc;
+    opts.plain_alloc = 0; // the plan owns its outputs
+    HIP_TRY(hipSetDevice(c->device));
+    memset(C, 0, (size_t)count * sizeof(*C));
+    hsmu_plan *p = new hsmu_plan();
+    p->c = c;
+    p->opts = opts;
+    memset(&p->stats0, 0, sizeof(p->stats0));
+    hipStream_t s = c->stream;
+    // recording run: eager, every block kept
+    {
+        Arena ar(c, &p->keep, false);
+        RunMode m;
+        m.plan = p;
+        rc = run_batch(c, count, A, B, C, opts, ar, m, &p->stats0, &p->empty);
+    }
+    if (!rc && !p->empty) {
+        if (hipMemcpyAsync(p->expect, p->rec, sizeof(p->rec), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = fail(HSMU_ERR_HIP, "hsmu_plan_create: uploading the checkpoints");
+    }
+    // capture pass: the same launches, the recorded sizes, no synchronisation
+    if (!rc && !p->empty) {
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            rc = fail(HSMU_ERR_HIP, "hsmu_plan_create: hipStreamBeginCapture");
+        } else {
+            std::vector<hsmu_csr> scratch_c((size_t)count);
+            Arena ar(c, &p->keep, true);
+            RunMode m;
+            m.plan = p;
+            m.capture = true;
+            bool empty;
+            rc = run_batch(c, count, A, B, scratch_c.data(), opts, ar, m, nullptr, &empty);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(s, &g);
+            if (!rc && e != hipSuccess)
+                rc = fail(HSMU_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+            p->graph = g;
+            if (!rc) {
+                const hipError_t ei = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+                if (ei != hipSuccess)
+                    rc = fail(HSMU_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+            }
+        }
+    }
+    if (rc) {
+        (void)hipGetLastError();
+        memset(C, 0, (size_t)count * sizeof(*C));
+        hsmu_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return HSMU_OK;
+}
+
+extern "C" int hsmu_plan_execute(hsmu_plan *p, hsmu_stats *stats)
+{
+    if (!p)
+        return fail(HSMU_ERR_INVALID, "hsmu_plan_execute: plan is NULL");
+    hsmu_ctx *c = p->c;
+    HIP_TRY(hipSetDevice(c->device));
+    if (stats)
+        *stats = p->stats0;
+    if (p->empty)
+        return HSMU_OK;
+    hipStream_t s = c->stream;
+    HIP_TRY(hipEventRecord(c->ev[EV_START], s));
+    HIP_TRY(hipGraphLaunch(p->exec, s));
+    HIP_TRY(hipEventRecord(c->ev[EV_NUMERIC], s));
+    HIP_TRY(hipMemcpyAsync(&c->host_stats->err, &p->dst->err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const int err = c->host_stats->err;
+    if (err & ERR_PLAN)
+        return fail(HSMU_ERR_PLAN, "the plan's inputs changed structure since it was made");
+    int rc = device_error_status(err);
+    if (rc)
+        return rc;
+    if (stats) {
+        for (int b = 0; b < NBINS; ++b)
+            stats->ms_numeric_bin[b] = 0.0;
+        stats->ms_conversion = stats->ms_form_tileCptr = stats->ms_tileOR = stats->ms_malloc = 0.0;
+        stats->ms_form_cval = 0.0;
+        stats->ms_total = elapsed(c->ev[EV_START], c->ev[EV_NUMERIC]);
+        stats->ms_time2 = stats->ms_total;
+        stats->gflops = stats->ms_total > 0 ? 2.0 * (double)stats->products / (stats->ms_total * 1e6) : 0.0;
     }
     return HSMU_OK;
 }

@@ -28,6 +28,36 @@ DEVI int xcd_block(int b, int n)
     return x * per + (x < rem ? x : rem) + q;
 }
 
+// Device work queue of the persistent row kernels: the n row groups of a launch are cut
+// into 8 contiguous ranges, one per XCD, each with a counter q[x] (zero at launch).  A
+// workgroup takes the next group of its own XCD's range (xq starts at blockIdx.x & 7) and,
+// once that range is used up, of the following ranges in turn, so the groups one XCD works
+// on stay neighbours (shared rows of B in its L2) until the queues drain.  Returns n when
+// no group is left.  Thread 0 pulls, the result goes to every thread through *s; the
+// caller's loop body must end with a workgroup barrier before the next pull.
+DEVI int pull_group(unsigned *q, int n, int &xq, int *s)
+{
+    if (threadIdx.x == 0) {
+        int got = n;
+        for (int k = 0; k < 8; ++k) {
+            const int x = (xq + k) & 7;
+            const int lo = (int)(((long long)n * x) >> 3), hi = (int)(((long long)n * (x + 1)) >> 3);
+            const unsigned len = (unsigned)(hi - lo);
+            if (__hip_atomic_load(&q[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len)
+                continue;
+            const unsigned v = atomicAdd(&q[x], 1u);
+            if (v < len) {
+                got = lo + (int)v;
+                xq = x;
+                break;
+            }
+        }
+        *s = got;
+    }
+    __syncthreads();
+    return *s;
+}
+
 // Sum over a group of G consecutive lanes.  G <= 64: shuffles; G > 64: through LDS
 // (every thread of the block must call it).
 template <int G, typename T> DEVI T group_sum(T v, T *red)

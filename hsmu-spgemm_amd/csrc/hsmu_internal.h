@@ -23,6 +23,7 @@ constexpr int ERR_ACOL = 2;     // column of A outside [0, rows of B)
 constexpr int ERR_BCOL = 4;     // column of B outside [0, N)
 constexpr int ERR_UNSORTED = 8; // row of B with decreasing columns
 constexpr int ERR_INTERNAL = 16;// symbolic and numeric counts disagree (a bug)
+constexpr int ERR_PLAN = 32;    // a replayed plan's inputs changed structure
 
 struct BinParams {
     long long lds_block_budget; // LDS bytes per workgroup for bins with several rows per workgroup
@@ -52,6 +53,26 @@ struct DevStats {
     int pad;
     BinStats sym;
     BinStats num;
+    // work-queue counters of the persistent LDS kernels: [phase][bin][XCD]
+    unsigned queue[2][NBINS][8];
+};
+
+// Batched form: instance i of a batch, as seen by the stacking kernels.  The stacked A
+// holds A_i's rows at mo.. and its entries at ea.., with columns shifted by ko (the first
+// row of B_i in the stacked B); the stacked B holds B_i's rows at ko.. and entries at
+// eb.., columns unchanged (so rows of the stacked C are the rows of C_i, columns local).
+struct BatchInst {
+    const int *Arp, *Acol;
+    const void *Aval;
+    const int *Brp, *Bcol;
+    const void *Bval;
+    int M, K, N, pad;
+};
+
+// Offsets of the instances, each array count+1 long (the last entry is the total):
+// rows of A (mo), rows of B (ko), entries of A (ea), entries of B (eb).
+struct BatchOffsets {
+    const long long *mo, *ko, *ea, *eb;
 };
 
 #if defined(__HIPCC__)
@@ -165,10 +186,12 @@ struct Operands {
 // Global-memory workspace (bin BIN_GLOBAL): gws_blocks regions of gws_stride u64 each;
 // a region holds gws_tiles 64-bit masks followed by gws_tiles 32-bit offsets, with
 // gws_tiles >= the widest tile span in the bin.
+// queue: nullptr = one workgroup per row group (grid from count); otherwise 8 zeroed
+// counters and the LDS kernels run persistent (grid from occupancy x num_cus).
 hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int count,
                            int max_need, const Operands &op, int *rnnz, int tshift,
                            unsigned long long *gws, long long gws_stride, int num_cus,
-                           int gws_blocks, hipStream_t s);
+                           int gws_blocks, unsigned *queue, hipStream_t s);
 
 hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *scan_ws,
                                long long *total_copy, hipStream_t s);
@@ -179,9 +202,36 @@ hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_
                           int max_need, const Operands &op, const long long *crp, int *Ccol,
                           void *Cval, float *comp, int bshift, int tshift, unsigned long long *gws,
                           long long gws_stride, long long gws_tiles, int gws_blocks, int *err,
-                          hipStream_t s);
+                          int num_cus, unsigned *queue, hipStream_t s);
 
 hipError_t launch_rowptr_i64_to_i32(const long long *in, int64_t n, int *out, hipStream_t s);
+
+// ---- batched form (kernels.hip) ----
+// Stacked row pointers of A (which = 0, nrows = total rows of A) or B (which = 1): entry g
+// of instance i = e0_i + rp_i[g - r0_i], the last entry = total entries.  Raises
+// ERR_ROWPTR when an instance's rp[0] != 0 or rp[rows] != its nnz.
+hipError_t launch_batch_rowptr(int which, int64_t count, const BatchInst *inst, BatchOffsets off,
+                               int64_t nrows, int64_t nnz, int *out, DevStats *st, hipStream_t s);
+// Stacked columns (A: + ko_i; B: unchanged) and values (val may be nullptr: not copied).
+// Raises ERR_ACOL / ERR_BCOL for a column outside instance i's range (and writes 0).
+hipError_t launch_batch_entries(int which, int64_t count, const BatchInst *inst, BatchOffsets off,
+                                int64_t nnz, int val32, int *col, void *val, DevStats *st,
+                                hipStream_t s);
+// Row pointers of every C_i from the stacked C's (crp, int64): instance i's M_i + 1 entries
+// at out + mo_i + i, rebased to start at 0; out is int64 (rp64) or int32.  Also
+// bases[i] = crp[mo_i] for i <= count (where C_i's entries start).  nout = rows + count.
+hipError_t launch_batch_split(int64_t count, BatchOffsets off, const long long *crp, void *out,
+                              int rp64, long long *bases, int64_t nout, hipStream_t s);
+
+// Replayed plans: compare the device's DevStats of one checkpoint (stage 0: after the
+// symbolic binning, stage 1: after the numeric binning) with the recorded one; on any
+// difference, or an input error already raised, set ERR_PLAN / keep err and zero the
+// bin counts of that stage so that no later kernel runs on sizes the plan did not
+// provision.
+hipError_t launch_plan_check(DevStats *st, const DevStats *expect, int stage, hipStream_t s);
+// Replayed batch plans: ERR_PLAN when the start of any C_i moved (its nnz changed).
+hipError_t launch_plan_bases(const long long *got, const long long *want, int64_t n, DevStats *st,
+                             hipStream_t s);
 
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cus, hipStream_t s);
 

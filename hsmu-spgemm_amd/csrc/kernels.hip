@@ -592,4 +592,176 @@ hipError_t launch_rowptr_i64_to_i32(const long long *in, int64_t n, int *out, hi
     return hipGetLastError();
 }
 
+// ---- batched form: stacking the instances and splitting C ----------------------------------
+
+// Largest i in [0, count) with off[i] + i * step <= x (off nondecreasing, off[0] = 0 <= x):
+// the instance whose range holds x, empty instances skipped.
+DEVI int find_instance(const long long *__restrict__ off, int count, long long x, int step)
+{
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] + (long long)mid * step <= x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_batch_rowptr(int which, int count, const BatchInst *__restrict__ inst,
+                                                      const long long *__restrict__ roff,
+                                                      const long long *__restrict__ eoff, int64_t nrows, int64_t nnz,
+                                                      int *__restrict__ out, DevStats *st)
+{
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g <= nrows; g += stride) {
+        if (g == nrows) {
+            out[g] = (int)nnz;
+            continue;
+        }
+        const int i = find_instance(roff, count, g, 0);
+        const BatchInst I = inst[i];
+        const int *rp = which ? I.Brp : I.Arp;
+        const long long r = g - roff[i];
+        const long long e0 = eoff[i];
+        out[g] = (int)(e0 + rp[r]);
+        if (r == 0) {
+            const int rows = which ? I.K : I.M;
+            if (rp[0] != 0 || (long long)rp[rows] != eoff[i + 1] - e0)
+                atomicOr(&st->err, ERR_ROWPTR);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch_entries(int which, int count, const BatchInst *__restrict__ inst,
+                                                       const long long *__restrict__ eoff,
+                                                       const long long *__restrict__ koff, int64_t nnz, int val32,
+                                                       int *__restrict__ col, void *__restrict__ val, DevStats *st)
+{
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    int bad = 0;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nnz; e += stride) {
+        const int i = find_instance(eoff, count, e, 0);
+        const BatchInst &I = inst[i];
+        const long long k = e - eoff[i];
+        const int c = which ? I.Bcol[k] : I.Acol[k];
+        const int lim = which ? I.N : I.K;
+        int cs = c;
+        if (c < 0 || c >= lim) {
+            bad |= which ? ERR_BCOL : ERR_ACOL;
+            cs = 0;
+        }
+        col[e] = which ? cs : (int)(cs + koff[i]);
+        if (val) {
+            const void *src = which ? I.Bval : I.Aval;
+            if (val32)
+                ((float *)val)[e] = ((const float *)src)[k];
+            else
+                ((double *)val)[e] = ((const double *)src)[k];
+        }
+    }
+    if (bad)
+        atomicOr(&st->err, bad);
+}
+
+__global__ __launch_bounds__(256) void k_batch_split(int count, const long long *__restrict__ mo,
+                                                     const long long *__restrict__ crp, void *__restrict__ out,
+                                                     int rp64, long long *__restrict__ bases)
+{
+    const int64_t n = mo[count] + count; // sum over instances of M_i + 1
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x; q < n; q += stride) {
+        const int i = find_instance(mo, count, q, 1);
+        const long long r0 = mo[i];
+        const long long v = crp[r0 + (q - r0 - i)] - crp[r0];
+        if (rp64)
+            ((long long *)out)[q] = v;
+        else
+            ((int *)out)[q] = (int)v;
+    }
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i <= count; i += 256)
+            bases[i] = crp[mo[i]];
+}
+
+static unsigned grid_for(int64_t n)
+{
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 16384)
+        blocks = 16384;
+    return (unsigned)(blocks < 1 ? 1 : blocks);
+}
+
+hipError_t launch_batch_rowptr(int which, int64_t count, const BatchInst *inst, BatchOffsets off, int64_t nrows,
+                               int64_t nnz, int *out, DevStats *st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_rowptr, dim3(grid_for(nrows + 1)), dim3(256), 0, s, which, (int)count, inst,
+                       which ? off.ko : off.mo, which ? off.eb : off.ea, nrows, nnz, out, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_entries(int which, int64_t count, const BatchInst *inst, BatchOffsets off, int64_t nnz,
+                                int val32, int *col, void *val, DevStats *st, hipStream_t s)
+{
+    if (nnz <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_batch_entries, dim3(grid_for(nnz)), dim3(256), 0, s, which, (int)count, inst,
+                       which ? off.eb : off.ea, off.ko, nnz, val32, col, val, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_split(int64_t count, BatchOffsets off, const long long *crp, void *out, int rp64,
+                              long long *bases, int64_t nout, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_batch_split, dim3(grid_for(nout)), dim3(256), 0, s, (int)count, off.mo, crp, out, rp64, bases);
+    return hipGetLastError();
+}
+
+// ---- replayed plans --------------------------------------------------------------------------
+
+__global__ void k_plan_check(DevStats *st, const DevStats *expect, int stage)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const BinStats &a = stage == 0 ? st->sym : st->num;
+    const BinStats &b = stage == 0 ? expect->sym : expect->num;
+    bool same = true;
+    for (int k = 0; k < NBINS; ++k)
+        same = same && a.count[k] == b.count[k] && a.max_need[k] == b.max_need[k] && a.max_span[k] == b.max_span[k];
+    if (stage == 0)
+        same = same && st->dir_total == expect->dir_total && st->ntiles_b == expect->ntiles_b;
+    else
+        same = same && st->nnz_c == expect->nnz_c;
+    if (!same)
+        st->err |= ERR_PLAN;
+    if (st->err) {
+        BinStats &w = stage == 0 ? st->sym : st->num;
+        for (int k = 0; k < NBINS; ++k)
+            w.count[k] = 0;
+    }
+}
+
+hipError_t launch_plan_check(DevStats *st, const DevStats *expect, int stage, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_plan_check, dim3(1), dim3(64), 0, s, st, expect, stage);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_plan_bases(const long long *__restrict__ got, const long long *__restrict__ want,
+                                                    int64_t n, DevStats *st)
+{
+    bool diff = false;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        diff = diff || got[i] != want[i];
+    if (diff)
+        atomicOr(&st->err, ERR_PLAN);
+}
+
+hipError_t launch_plan_bases(const long long *got, const long long *want, int64_t n, DevStats *st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_plan_bases, dim3(grid_for(n)), dim3(256), 0, s, got, want, n, st);
+    return hipGetLastError();
+}
+
 } // namespace hsmu

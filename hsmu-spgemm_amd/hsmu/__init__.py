@@ -34,6 +34,7 @@ HSMU_ERR_OVERFLOW = 4
 HSMU_ERR_NOMEM = 5
 HSMU_ERR_HIP = 6
 HSMU_ERR_UNSUPPORTED = 7
+HSMU_ERR_PLAN = 8
 HSMU_FP64 = 0
 HSMU_FP32_COMP = 1
 NBINS = 10
@@ -67,7 +68,7 @@ class Opts(C.Structure):
                 ("force_global", C.c_int32), ("lds_row_max", C.c_int32), ("lds_block_budget", C.c_int32),
                 ("products_per_lane", C.c_int32), ("tiles_per_lane", C.c_int32), ("timing", C.c_int32),
                 ("b_lanes", C.c_int32), ("t_lanes", C.c_int32), ("rebuild_dirs", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("persistent", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Stats(C.Structure):
@@ -103,6 +104,16 @@ _sig = {
     "hsmu_spgemm": (C.c_int, [_P, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Opts),
                               C.POINTER(Stats)]),
     "hsmu_csr_free": (C.c_int, [_P, C.POINTER(Csr)]),
+    "hsmu_spgemm_batched": (C.c_int, [_P, C.c_int64, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr),
+                                      C.POINTER(Opts), C.POINTER(Stats)]),
+    "hsmu_batch_free": (C.c_int, [_P, C.c_int64, C.POINTER(Csr)]),
+    "hsmu_plan_create": (C.c_int, [_P, C.c_int64, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr),
+                                   C.POINTER(Opts), C.POINTER(_P)]),
+    "hsmu_plan_execute": (C.c_int, [_P, C.POINTER(Stats)]),
+    "hsmu_plan_destroy": (None, [_P]),
+    "hsmu_shape_class": (C.c_int, [C.c_int64, C.POINTER(Csr), C.POINTER(Csr), C.c_char_p, C.c_size_t]),
+    "hsmu_autotune": (C.c_int, [_P, C.c_int64, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Opts), C.c_char_p,
+                                C.c_int, C.POINTER(Opts), C.POINTER(C.c_int32)]),
     "hsmu_btiles": (C.c_int, [_P, C.POINTER(Csr), C.POINTER(Tiles)]),
     "hsmu_tiles_free": (C.c_int, [_P, C.POINTER(Tiles)]),
     "hsmu_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
@@ -348,6 +359,46 @@ class Context:
             dA.free()
             dB.free()
 
+    # ---- batched form ----
+
+    @staticmethod
+    def _pair_arrays(As, Bs):
+        n = len(As)
+        if n != len(Bs) or n < 1:
+            raise ValueError("As and Bs must be non-empty lists of the same length")
+        a = (Csr * n)(*[x.csr for x in As])
+        b = (Csr * n)(*[x.csr for x in Bs])
+        return n, a, b
+
+    def spgemm_batched_device(self, As, Bs, opts: Opts | None = None):
+        """C_i = A_i * B_i for every pair, as one pipeline over all instances.  Returns
+        (array of Csr views, stats dict); release with free_batch."""
+        n, a, b = self._pair_arrays(As, Bs)
+        cs = (Csr * n)()
+        st = Stats()
+        o = opts if opts is not None else default_opts()
+        _check(lib.hsmu_spgemm_batched(self.handle, n, a, b, cs, C.byref(o), C.byref(st)), "hsmu_spgemm_batched")
+        return cs, st.to_dict()
+
+    def free_batch(self, cs) -> None:
+        _check(lib.hsmu_batch_free(self.handle, len(cs), cs), "hsmu_batch_free")
+
+    def plan(self, As, Bs, opts: Opts | None = None) -> "Plan":
+        """Record the batch (or a single product: lists of one) and capture it as a HIP
+        graph; Plan.execute() replays it on the current input values."""
+        return Plan(self, As, Bs, opts)
+
+    def autotune(self, As, Bs, base: Opts | None = None, table: str | None = None, search: bool = True):
+        """(opts, found_in_table) for the shape class of the batch (lists of one for a
+        single product)."""
+        n, a, b = self._pair_arrays(As, Bs)
+        out = Opts()
+        found = C.c_int32(0)
+        bo = base if base is not None else default_opts()
+        _check(lib.hsmu_autotune(self.handle, n, a, b, C.byref(bo), table.encode() if table else None,
+                                 1 if search else 0, C.byref(out), C.byref(found)), "hsmu_autotune")
+        return out, bool(found.value)
+
     def btiles(self, B: HostMatrix) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
         """B -> (tptr, tcol, tmask) in the compressed 64-column tile format."""
         dB = self.matrix(B)
@@ -376,3 +427,44 @@ class Context:
             lib.hsmu_device_free(self.handle, a)
             lib.hsmu_device_free(self.handle, b)
         return ms.value
+
+
+def shape_class(As, Bs) -> str:
+    """Autotuner shape class of a batch (lists of DeviceMatrix; lists of one for a
+    single product)."""
+    n, a, b = Context._pair_arrays(As, Bs)
+    buf = C.create_string_buffer(128)
+    _check(lib.hsmu_shape_class(n, a, b, buf, 128), "hsmu_shape_class")
+    return buf.value.decode()
+
+
+class Plan:
+    """hsmu_plan: a recorded batch replayed as one HIP graph.  ``results`` are the Csr
+    views of C_i (owned by the plan, valid until destroy)."""
+
+    def __init__(self, ctx: Context, As, Bs, opts: Opts | None = None):
+        self.ctx = ctx
+        n, a, b = Context._pair_arrays(As, Bs)
+        self._keep = (a, b)
+        self.results = (Csr * n)()
+        h = _P()
+        o = opts if opts is not None else default_opts()
+        _check(lib.hsmu_plan_create(ctx.handle, n, a, b, self.results, C.byref(o), C.byref(h)), "hsmu_plan_create")
+        self.handle = h
+
+    def execute(self) -> dict:
+        st = Stats()
+        _check(lib.hsmu_plan_execute(self.handle, C.byref(st)), "hsmu_plan_execute")
+        return st.to_dict()
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib.hsmu_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.ctx.handle:
+                self.destroy()
+        except Exception:
+            pass

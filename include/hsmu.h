@@ -49,7 +49,8 @@ enum hsmu_status {
     HSMU_ERR_OVERFLOW = 4,    /* nnz(C) does not fit the requested row-pointer type */
     HSMU_ERR_NOMEM = 5,       /* device or host allocation failed */
     HSMU_ERR_HIP = 6,         /* HIP runtime error (message in hsmu_last_error) */
-    HSMU_ERR_UNSUPPORTED = 7  /* valid request this build does not implement */
+    HSMU_ERR_UNSUPPORTED = 7, /* valid request this build does not implement */
+    HSMU_ERR_PLAN = 8         /* a plan's inputs no longer have the structure it was made for */
 };
 
 /* numeric form */
@@ -119,7 +120,10 @@ typedef struct hsmu_opts {
                                  of two, 1..64); 0 = from the mean tiles per row of B */
     int32_t rebuild_dirs;     /* 1: the numeric phase rebuilds each row's tile directory
                                  instead of reusing the one the symbolic phase kept */
-    int32_t reserved[4];
+    int32_t persistent;       /* 1: the LDS row kernels run as persistent workgroups (as
+                                 many as fit the device) pulling row groups from per-XCD
+                                 device work queues; 0: one workgroup per row group */
+    int32_t reserved[3];
 } hsmu_opts;
 
 #define HSMU_NBINS 10 /* 8 LDS bins (8..1024 lanes per row), global fallback, wide rows */
@@ -183,6 +187,54 @@ int hsmu_csr_free(hsmu_ctx *ctx, hsmu_csr *C);
 /* B -> compressed 64-column tiles (the conversion stage on its own). */
 int hsmu_btiles(hsmu_ctx *ctx, const hsmu_csr *B, hsmu_tiles *T);
 int hsmu_tiles_free(hsmu_ctx *ctx, hsmu_tiles *T);
+
+/* ---- batched form (no counterpart in the reference, which multiplies one pair per
+ *      process: src/test.cu:10-94) ----------------------------------------------------- */
+
+/* C[i] = A[i] * B[i] for i < count, as ONE pipeline over all instances: the instances
+ * are stacked (rows of every A[i] one after another, columns of A[i] shifted onto the
+ * rows of B[i] in a stacked B), so every stage is one launch over the rows of all
+ * instances and the bins mix rows of different instances.  Two host synchronisations
+ * for the whole batch.  Instances may differ in every size.
+ * Output: C[i] are views into three device arrays owned by the batch; C[0]'s pointers
+ * are their bases.  Row pointers of every C[i] start at 0.  Release with
+ * hsmu_batch_free (never hsmu_csr_free on a C[i]).  stats (may be NULL) covers the
+ * whole batch.  With int32 row pointers the whole batch's nnz must fit int32. */
+int hsmu_spgemm_batched(hsmu_ctx *ctx, int64_t count, const hsmu_csr *A, const hsmu_csr *B,
+                        hsmu_csr *C, const hsmu_opts *opts, hsmu_stats *stats);
+int hsmu_batch_free(hsmu_ctx *ctx, int64_t count, hsmu_csr *C);
+
+/* ---- plans: inspect once, replay as one HIP graph ----------------------------------
+ * hsmu_plan_create runs the batch (count >= 1; count == 1 is a single SpGEMM) once,
+ * records every launch size, LDS size and buffer it used, and captures the whole
+ * sequence (input stacking, B -> tiles, row analysis, bins, symbolic, scans, numeric,
+ * output row pointers) into a HIP graph that contains no host synchronisation, no
+ * allocation and no memset.  C[i] are filled by the create call and stay owned by the
+ * plan.  hsmu_plan_execute replays the graph: C is recomputed from the CURRENT contents
+ * of the bound input arrays, whose values may change but whose structure (row pointers
+ * and columns) must not; a changed structure is detected on the device and reported as
+ * HSMU_ERR_PLAN.  stats (may be NULL) gets ms_total = the replay's device time. */
+typedef struct hsmu_plan hsmu_plan;
+int hsmu_plan_create(hsmu_ctx *ctx, int64_t count, const hsmu_csr *A, const hsmu_csr *B,
+                     hsmu_csr *C, const hsmu_opts *opts, hsmu_plan **plan);
+int hsmu_plan_execute(hsmu_plan *plan, hsmu_stats *stats);
+void hsmu_plan_destroy(hsmu_plan *plan); /* releases the plan's buffers, C[i] included */
+
+/* ---- autotuner ------------------------------------------------------------------------
+ * Bin thresholds (products_per_lane, tiles_per_lane), sub-group widths (b_lanes,
+ * t_lanes), the LDS budget per workgroup and the persistent mode, chosen per SHAPE CLASS
+ * of the input: the class is "c<C>_a<Ea>_b<Eb>_m<Em>" from the instance count bucket C
+ * (1, or log2 of the count), the rounded log2 of the mean row length of A and of B, and
+ * the floor of log2 of the total rows of A (hsmu_shape_class).  The table is a text
+ * file, one line per class: class, the six values, the measured ms.  When the class is
+ * in the table its values are returned; otherwise, if search is nonzero, the candidates
+ * are timed on this input (coordinate descent from *base, each candidate run and timed
+ * three times, best median kept), the winner is appended to the table (if table_path is
+ * not NULL) and returned; if search is zero, *base is returned.  *found = 1 when the class came from the table. */
+int hsmu_shape_class(int64_t count, const hsmu_csr *A, const hsmu_csr *B, char *cls, size_t cls_len);
+int hsmu_autotune(hsmu_ctx *ctx, int64_t count, const hsmu_csr *A, const hsmu_csr *B,
+                  const hsmu_opts *base, const char *table_path, int search, hsmu_opts *out,
+                  int32_t *found);
 
 /* ---- device memory helpers (for bindings that carry no device allocator) -------- */
 int hsmu_device_alloc(hsmu_ctx *ctx, size_t bytes, void **ptr);
