@@ -51,11 +51,25 @@ DEVI void stage_row(const Operands &op, int a0, int nA, int *t0, int *tlen, int 
 
 constexpr int UNROLL = 4;
 
-// OR the tiles of B_j into dm for every entry j of the row: sub-groups of 2^tshift
-// lanes take one entry each and walk its tiles, UNROLL tiles per lane per iteration
-// with all loads issued before the ds_or_b64s.
-template <int G>
-DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, u64 *dm, int mn, int tshift)
+// Compact index of tile t (relative to rmin) in a two-level directory: the rank of t
+// among the row's non-empty tiles.
+DEVI unsigned fine_index(const u64 *cm, const unsigned *coff, int t)
+{
+    const int w = t >> 6;
+    return coff[w] + (unsigned)__popcll(cm[w] & below(t & 63));
+}
+
+// Walks over the row's tile-products (every tile of every B_j of the row):
+//   OR_FLAT    dm[t] |= mask                  flat directory
+//   OR_COARSE  cm[t >> 6] |= bit (t & 63)     two-level, pass 1: which tiles are non-empty
+//   OR_FINE    fm[fine_index(t)] |= mask      two-level, pass 2
+enum { OR_FLAT, OR_COARSE, OR_FINE };
+
+// Sub-groups of 2^tshift lanes take one entry j each and walk the tiles of B_j, UNROLL
+// tiles per lane per iteration with all loads issued before the ds_or_b64s.
+template <int G, int MODE>
+DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, u64 *dm, const unsigned *coff, u64 *fm,
+                   int mn, int tshift)
 {
     const int l = threadIdx.x & (G - 1);
     const int SG = 1 << tshift, sl = l & (SG - 1), NSG = G >> tshift;
@@ -71,15 +85,40 @@ DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, u
                 m[u] = 0;
                 if (tu < te) {
                     c[u] = op.tcol[tu];
-                    m[u] = op.tmask[tu];
+                    if constexpr (MODE != OR_COARSE)
+                        m[u] = op.tmask[tu];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u)
-                if (c[u] >= 0)
-                    atomicOr(&dm[c[u] - mn], m[u]);
+            for (int u = 0; u < UNROLL; ++u) {
+                if (c[u] >= 0) {
+                    const int tt = c[u] - mn;
+                    if constexpr (MODE == OR_FLAT)
+                        atomicOr(&dm[tt], m[u]);
+                    else if constexpr (MODE == OR_COARSE)
+                        atomicOr(&dm[tt >> 6], 1ull << (tt & 63));
+                    else
+                        atomicOr(&fm[fine_index(dm, coff, tt)], m[u]);
+                }
+            }
         }
     }
+}
+
+// Each lane scans a contiguous chunk of the n directory entries: the group's exclusive
+// prefix of the popcounts at the chunk start, the group total in tot.  Every thread of the
+// workgroup must call it (G > 64 synchronises the workgroup).
+template <int G>
+DEVI unsigned scan_popcounts(const u64 *m, int n, unsigned *redu, unsigned &tot, int &ta, int &tb)
+{
+    const int l = threadIdx.x & (G - 1);
+    const int chunk = (n + G - 1) / G;
+    ta = l * chunk;
+    tb = ta + chunk < n ? ta + chunk : n;
+    unsigned loc = 0;
+    for (int t = ta; t < tb; ++t)
+        loc += (unsigned)__popcll(m[t]);
+    return group_excl_scan<G>(loc, redu, tot);
 }
 
 // ---- numeric forms ---------------------------------------------------------------------
@@ -152,12 +191,16 @@ struct PrecF32C {
     }
 };
 
+// Slot of column c in the row's accumulator, by directory form.
+enum { SLOT_FLAT, SLOT_DENSE, SLOT_TWO_LEVEL };
+
 // Add every product a_ij * b_jc of the row into its slot: sub-groups of 2^bshift lanes
 // take one entry each and walk B_j, UNROLL elements per lane per iteration with all
-// loads issued before the ds_add_f64s.
-template <int G, bool DENSE, typename P>
-DEVI void add_products(const Operands &op, const int *b0, const int *blen, const double *av, int nA, const u64 *dm,
-                       const unsigned *doff, double *vals, int mn, int bshift)
+// loads issued before the ds_add_f64s.  fm/foff: the (fine) tile masks and first slots;
+// cm/coff: the coarse level of a two-level directory.
+template <int G, int FORM, typename P>
+DEVI void add_products(const Operands &op, const int *b0, const int *blen, const double *av, int nA, const u64 *cm,
+                       const unsigned *coff, const u64 *fm, const unsigned *foff, double *vals, int mn, int bshift)
 {
     typedef typename P::V V;
     const V *__restrict__ Bval = (const V *)op.Bval;
@@ -183,11 +226,12 @@ DEVI void add_products(const Operands &op, const int *b0, const int *blen, const
             for (int u = 0; u < UNROLL; ++u) {
                 if (c[u] >= 0) {
                     unsigned slot;
-                    if constexpr (DENSE) {
+                    if constexpr (FORM == SLOT_DENSE) {
                         slot = (unsigned)(c[u] - (mn << 6));
                     } else {
                         const int t = (c[u] >> 6) - mn;
-                        slot = doff[t] + (unsigned)__popcll(dm[t] & below(c[u] & 63));
+                        const unsigned k = FORM == SLOT_TWO_LEVEL ? fine_index(cm, coff, t) : (unsigned)t;
+                        slot = foff[k] + (unsigned)__popcll(fm[k] & below(c[u] & 63));
                     }
                     P::add(vals, slot, a, v[u]);
                 }
@@ -198,66 +242,106 @@ DEVI void add_products(const Operands &op, const int *b0, const int *blen, const
 
 // ---- symbolic ------------------------------------------------------------------------------
 
-// nnz(C_i) for rows kept in LDS: stage the row, OR the tile masks with ds_or_b64 into a
-// dense directory over the row's tile span, popcount.
+// nnz(C_i) for rows kept in LDS: stage the row, OR the tile masks with ds_or_b64 into the
+// row's tile directory, popcount.  Flat rows: one pass into a directory over the whole
+// span.  Two-level rows: pass 1 marks the non-empty tiles in the coarse bitmap, a scan
+// gives their compact indices, pass 2 ORs the masks into the compact fine level.
+// two_level = 0 when no row of the launch is two-level (its extra steps are skipped).
+// When the directory is kept, it goes out with each entry's exclusive prefix (coarse:
+// first compact index; fine or flat: first slot) for the numeric phase.
 template <int G>
 __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const int *__restrict__ rows,
                                                                        const int *__restrict__ count_p, int slice_bytes,
                                                                        Operands op, int *__restrict__ rnnz, int tshift,
-                                                                       unsigned *queue)
+                                                                       int two_level, unsigned *queue)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned redu[16];
-    __shared__ int grp_s;
+    __shared__ int grp_s[2];
     const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
     unsigned char *slice = lds + (size_t)g * slice_bytes;
     const int nrows = *count_p;
-    const int ngroups = (nrows + RPB - 1) / RPB;
-    int xq = blockIdx.x & 7;
-    for (int grp = queue ? pull_group(queue, ngroups, xq, &grp_s) : xcd_block(blockIdx.x, gridDim.x); grp < ngroups;
-         grp = queue ? pull_group(queue, ngroups, xq, &grp_s) : grp + (int)gridDim.x) {
-        const int idx = grp * RPB + g;
+    for (GroupWalk gw(queue, (nrows + RPB - 1) / RPB, grp_s); gw.more(); gw.next()) {
+        const int idx = gw.grp * RPB + g;
         const int row = idx < nrows ? rows[idx] : -1;
-        int S = 0, mn = 0, a0 = 0, nA = 0;
+        int S = 0, mn = 0, a0 = 0, nA = 0, tp = 0;
         if (row >= 0) {
             mn = op.rmin[row];
             S = op.rmax[row] - mn + 1;
             a0 = op.Arp[row];
             nA = op.Arp[row + 1] - a0;
+            tp = op.rtp[row];
         }
-        const RowLayout L = row_layout(0, S, 0, nA);
+        const bool sparse = use_sparse(S, tp);
+        const int W = sparse ? (S + 63) >> 6 : S; // entries of the first level
+        const RowLayout L = row_layout(0, S, 0, nA, tp);
         u64 *dm = (u64 *)(slice + L.dm);
+        unsigned *doff = (unsigned *)(slice + L.doff);
         int *tlen = (int *)(slice + L.tlen);
         int *t0 = (int *)(slice + L.t0);
-        for (int t = l; t < S; t += G)
+        for (int t = l; t < W; t += G)
             dm[t] = 0;
         stage_row<G, false>(op, a0, nA, t0, tlen, nullptr, nullptr, nullptr);
         __syncthreads();
-        or_tiles<G>(op, t0, tlen, nA, dm, mn, tshift);
+        if (sparse)
+            or_tiles<G, OR_COARSE>(op, t0, tlen, nA, dm, nullptr, nullptr, mn, tshift);
+        else
+            or_tiles<G, OR_FLAT>(op, t0, tlen, nA, dm, nullptr, nullptr, mn, tshift);
         __syncthreads();
-        // nnz(C_i) = sum of popcounts; when the directory is kept, the exclusive prefix
-        // (slot of each tile's first column) goes out with the masks for the numeric phase
-        const int chunk = (S + G - 1) / G;
-        const int ta = l * chunk;
-        const int tb = ta + chunk < S ? ta + chunk : S;
-        unsigned loc = 0;
-        for (int t = ta; t < tb; ++t)
-            loc += (unsigned)__popcll(dm[t]);
+        int ta, tb;
         unsigned tot;
-        unsigned pre = group_excl_scan<G>(loc, redu, tot);
+        unsigned pre = scan_popcounts<G>(dm, W, redu, tot, ta, tb);
         const long long rd = (row >= 0 && op.dir_mask) ? op.rdir[row] : -1;
-        if (rd >= 0) {
+        unsigned nnz = tot;
+        if (!sparse) {
+            if (rd >= 0) {
+                for (int t = ta; t < tb; ++t) {
+                    const u64 m = dm[t];
+                    op.dir_mask[rd + t] = m;
+                    op.dir_off[rd + t] = pre;
+                    pre += (unsigned)__popcll(m);
+                }
+            }
+        } else {
             for (int t = ta; t < tb; ++t) {
                 const u64 m = dm[t];
-                op.dir_mask[rd + t] = m;
-                op.dir_off[rd + t] = pre;
+                doff[t] = pre;
+                if (rd >= 0) {
+                    op.dir_mask[rd + t] = m;
+                    op.dir_off[rd + t] = pre;
+                }
                 pre += (unsigned)__popcll(m);
             }
         }
+        if (two_level) {
+            // two-level rows: tot = non-empty tiles; build and count the fine level
+            u64 *fm = dm + W;
+            __syncthreads();
+            if (sparse)
+                for (int k = l; k < (int)tot; k += G)
+                    fm[k] = 0;
+            __syncthreads();
+            if (sparse)
+                or_tiles<G, OR_FINE>(op, t0, tlen, nA, dm, doff, fm, mn, tshift);
+            __syncthreads();
+            unsigned tot2;
+            unsigned pre2 = scan_popcounts<G>(fm, sparse ? (int)tot : 0, redu, tot2, ta, tb);
+            if (sparse) {
+                nnz = tot2;
+                if (rd >= 0) {
+                    for (int k = ta; k < tb; ++k) {
+                        const u64 m = fm[k];
+                        op.dir_mask[rd + W + k] = m;
+                        op.dir_off[rd + W + k] = pre2;
+                        pre2 += (unsigned)__popcll(m);
+                    }
+                }
+            }
+        }
         if (row >= 0 && l == 0)
-            rnnz[row] = (int)tot;
+            rnnz[row] = (int)nnz;
         __syncthreads();
     }
 }
@@ -300,34 +384,33 @@ __global__ __launch_bounds__(1024) void k_symbolic_global(const int *__restrict_
 
 // ---- numeric -------------------------------------------------------------------------------
 
-// The HSMU accumulator in LDS (slice layout: row_layout).  After staging, the row's tile
-// directory is rebuilt as in the symbolic phase, an exclusive scan of its popcounts
-// gives every tile's first slot, every product is added with ds_add_f64 into its slot,
-// and the row of C is written in column order: for G >= 64 one wave per tile, lane b
-// writing column 64*(rmin+t)+b if present (coalesced); for G < 64 one lane per tile
-// walking its set bits.
+// The HSMU accumulator in LDS (slice layout: row_layout).  The row's tile directory is
+// loaded (kept by the symbolic phase) or rebuilt as there, every product is added with
+// ds_add_f64 into its slot, and the row of C is written in column order: for G >= 64 one
+// wave per tile, lane b writing column 64*(rmin+t)+b if present (coalesced); for G < 64
+// one lane per tile walking its set bits.  Directory forms: flat (slot from the tile's
+// entry), dense (one slot per column of the span) and two-level (the tile's entry found
+// through the coarse bitmap first); two_level = 0 when no row of the launch uses it.
 template <int G, typename P>
 __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int *__restrict__ rows,
                                                                       const int *__restrict__ count_p, int slice_bytes,
                                                                       Operands op, const long long *__restrict__ crp,
                                                                       int *__restrict__ Ccol, typename P::V *__restrict__ Cval,
-                                                                      int bshift, int tshift, int *err, unsigned *queue)
+                                                                      int bshift, int tshift, int two_level, int *err,
+                                                                      unsigned *queue)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ unsigned redu[16];
-    __shared__ int grp_s;
+    __shared__ int grp_s[2];
     const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
     unsigned char *slice = lds + (size_t)g * slice_bytes;
     const int nrows = *count_p;
-    const int ngroups = (nrows + RPB - 1) / RPB;
-    int xq = blockIdx.x & 7;
-    for (int grp = queue ? pull_group(queue, ngroups, xq, &grp_s) : xcd_block(blockIdx.x, gridDim.x); grp < ngroups;
-         grp = queue ? pull_group(queue, ngroups, xq, &grp_s) : grp + (int)gridDim.x) {
-        const int idx = grp * RPB + g;
+    for (GroupWalk gw(queue, (nrows + RPB - 1) / RPB, grp_s); gw.more(); gw.next()) {
+        const int idx = gw.grp * RPB + g;
         const int row = idx < nrows ? rows[idx] : -1;
-        int S = 0, mn = 0, n = 0, a0 = 0, nA = 0;
+        int S = 0, mn = 0, n = 0, a0 = 0, nA = 0, tp = 0;
         long long cb = 0;
         if (row >= 0) {
             mn = op.rmin[row];
@@ -336,80 +419,130 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
             n = (int)(crp[row + 1] - cb);
             a0 = op.Arp[row];
             nA = op.Arp[row + 1] - a0;
+            tp = op.rtp[row];
         }
-        const RowLayout L = row_layout(1, S, n, nA);
+        const bool sparse = use_sparse(S, tp);
+        const int W = sparse ? (S + 63) >> 6 : S; // entries of the first level
+        const RowLayout L = row_layout(1, S, n, nA, tp);
         u64 *dm = (u64 *)(slice + L.dm);
         unsigned *doff = (unsigned *)(slice + L.doff);
+        unsigned *ft = (unsigned *)(slice + L.ft);
         double *vals = (double *)(slice + L.vals);
         double *av = (double *)(slice + L.av);
         int *tlen = (int *)(slice + L.tlen);
         int *t0 = (int *)(slice + L.t0);
         int *blen = (int *)(slice + L.blen);
         int *b0 = (int *)(slice + L.b0);
-        const bool dense = use_dense(S, n);
+        // fine level: the flat directory itself, or the compact entries after the coarse words
+        u64 *fm = sparse ? dm + W : dm;
+        unsigned *foff = sparse ? doff + W : doff;
+        const bool dense = !sparse && use_dense(S, n);
         const int nv = dense ? 64 * S : n;
         const long long rd = (row >= 0 && op.dir_mask) ? op.rdir[row] : -1;
         for (int q = l; q < nv; q += G)
             vals[q] = 0.0;
         if (rd >= 0) {
-            // directory kept by the symbolic phase: load it, stage only B's ranges
-            for (int t = l; t < S; t += G) {
+            // directory kept by the symbolic phase: load its first level, stage only B's ranges
+            for (int t = l; t < W; t += G) {
                 dm[t] = op.dir_mask[rd + t];
                 doff[t] = op.dir_off[rd + t];
             }
             stage_row<G, true>(op, a0, nA, nullptr, nullptr, b0, blen, av);
         } else {
-            for (int t = l; t < S; t += G)
+            for (int t = l; t < W; t += G)
                 dm[t] = 0;
             stage_row<G, true>(op, a0, nA, t0, tlen, b0, blen, av);
         }
         __syncthreads();
-        if (rd < 0)
-            or_tiles<G>(op, t0, tlen, nA, dm, mn, tshift);
+        if (rd < 0) {
+            if (sparse)
+                or_tiles<G, OR_COARSE>(op, t0, tlen, nA, dm, nullptr, nullptr, mn, tshift);
+            else
+                or_tiles<G, OR_FLAT>(op, t0, tlen, nA, dm, nullptr, nullptr, mn, tshift);
+        }
         __syncthreads();
-        // slot offsets (rebuilt directories) and the nnz check (all rows): each lane
-        // scans a contiguous chunk of tiles
-        const int chunk = (S + G - 1) / G;
-        const int ta = l * chunk;
-        const int tb = ta + chunk < S ? ta + chunk : S;
-        unsigned loc = 0;
-        for (int t = ta; t < tb; ++t)
-            loc += (unsigned)__popcll(dm[t]);
+        // first level: flat slot offsets (rebuilt) or coarse compact indices (rebuilt), and
+        // the total (flat: nnz(C_i); two-level: non-empty tiles)
+        int ta, tb;
         unsigned tot;
-        unsigned pre = group_excl_scan<G>(loc, redu, tot);
+        unsigned pre = scan_popcounts<G>(dm, W, redu, tot, ta, tb);
         if (rd < 0) {
             for (int t = ta; t < tb; ++t) {
                 doff[t] = pre;
                 pre += (unsigned)__popcll(dm[t]);
             }
         }
-        const bool ok = (int)tot == n;
+        unsigned nnz = tot;
+        const int F = sparse ? (int)tot : S; // fine entries
+        if (two_level) {
+            __syncthreads();
+            if (sparse) {
+                if (rd >= 0) {
+                    for (int k = l; k < F; k += G) {
+                        fm[k] = op.dir_mask[rd + W + k];
+                        foff[k] = op.dir_off[rd + W + k];
+                    }
+                } else {
+                    for (int k = l; k < F; k += G)
+                        fm[k] = 0;
+                }
+                // tile of every fine entry, from the coarse bitmap
+                for (int w = l; w < W; w += G) {
+                    u64 m = dm[w];
+                    unsigned k = doff[w];
+                    while (m) {
+                        ft[k++] = (unsigned)((w << 6) + __ffsll((unsigned long long)m) - 1);
+                        m &= m - 1;
+                    }
+                }
+            }
+            __syncthreads();
+            if (sparse && rd < 0)
+                or_tiles<G, OR_FINE>(op, t0, tlen, nA, dm, doff, fm, mn, tshift);
+            __syncthreads();
+            unsigned tot2;
+            unsigned pre2 = scan_popcounts<G>(fm, sparse ? F : 0, redu, tot2, ta, tb);
+            if (sparse) {
+                nnz = tot2;
+                if (rd < 0) {
+                    for (int k = ta; k < tb; ++k) {
+                        foff[k] = pre2;
+                        pre2 += (unsigned)__popcll(fm[k]);
+                    }
+                }
+            }
+        }
+        const bool ok = (int)nnz == n;
         if (row >= 0 && !ok && l == 0)
             atomicOr(err, ERR_INTERNAL);
         __syncthreads();
         if (row >= 0 && ok) {
             if (dense)
-                add_products<G, true, P>(op, b0, blen, av, nA, dm, doff, vals, mn, bshift);
+                add_products<G, SLOT_DENSE, P>(op, b0, blen, av, nA, nullptr, nullptr, fm, foff, vals, mn, bshift);
+            else if (sparse)
+                add_products<G, SLOT_TWO_LEVEL, P>(op, b0, blen, av, nA, dm, doff, fm, foff, vals, mn, bshift);
             else
-                add_products<G, false, P>(op, b0, blen, av, nA, dm, doff, vals, mn, bshift);
+                add_products<G, SLOT_FLAT, P>(op, b0, blen, av, nA, nullptr, nullptr, fm, foff, vals, mn, bshift);
         }
         __syncthreads();
         if (row >= 0 && ok) {
             if constexpr (G >= 64) {
                 constexpr int NW = G / 64;
                 const int w = l >> 6, lane = l & 63;
-                for (int t = w; t < S; t += NW) {
-                    const u64 m = dm[t];
+                for (int k = w; k < F; k += NW) {
+                    const u64 m = fm[k];
                     if ((m >> lane) & 1ull) {
-                        const unsigned pos = doff[t] + (unsigned)__popcll(m & below(lane));
+                        const int t = sparse ? (int)ft[k] : k;
+                        const unsigned pos = foff[k] + (unsigned)__popcll(m & below(lane));
                         Ccol[cb + pos] = ((mn + t) << 6) + lane;
                         Cval[cb + pos] = P::result(vals, dense ? (t << 6) + lane : pos);
                     }
                 }
             } else {
-                for (int t = l; t < S; t += G) {
-                    u64 m = dm[t];
-                    unsigned pos = doff[t];
+                for (int k = l; k < F; k += G) {
+                    u64 m = fm[k];
+                    unsigned pos = foff[k];
+                    const int t = sparse ? (int)ft[k] : k;
                     const int c0 = (mn + t) << 6;
                     while (m) {
                         const int b = __ffsll((unsigned long long)m) - 1;
@@ -548,7 +681,7 @@ static int64_t row_grid(Kern kernel, int bs, size_t lds, int64_t groups, int num
 
 template <int G>
 static void symbolic_g(const int *rows, const int *count_dev, int count, int max_need, const Operands &op, int *rnnz,
-                       int tshift, int num_cus, unsigned *queue, hipStream_t s)
+                       int tshift, int two_level, int num_cus, unsigned *queue, hipStream_t s)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
@@ -557,24 +690,24 @@ static void symbolic_g(const int *rows, const int *count_dev, int count, int max
     const int64_t blocks = row_grid(k_symbolic_lds<G>, BS, lds, ((int64_t)count + RPB - 1) / RPB, num_cus, queue);
     const int lg = __builtin_ctz(G);
     hipLaunchKernelGGL((k_symbolic_lds<G>), dim3((unsigned)blocks), dim3(BS), lds, s, rows, count_dev, slice, op, rnnz,
-                       tshift > lg ? lg : tshift, queue);
+                       tshift > lg ? lg : tshift, two_level, queue);
 }
 
 hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
-                           int *rnnz, int tshift, u64 *gws, long long gws_stride, int num_cus, int gws_blocks,
-                           unsigned *queue, hipStream_t s)
+                           int *rnnz, int tshift, int two_level, u64 *gws, long long gws_stride, int num_cus,
+                           int gws_blocks, unsigned *queue, hipStream_t s)
 {
     if (count <= 0)
         return hipSuccess;
     switch (bin) {
-    case 0: symbolic_g<8>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
-    case 1: symbolic_g<16>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
-    case 2: symbolic_g<32>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
-    case 3: symbolic_g<64>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
-    case 4: symbolic_g<128>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
-    case 5: symbolic_g<256>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
-    case 6: symbolic_g<512>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
-    case 7: symbolic_g<1024>(rows, count_dev, count, max_need, op, rnnz, tshift, num_cus, queue, s); break;
+    case 0: symbolic_g<8>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
+    case 1: symbolic_g<16>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
+    case 2: symbolic_g<32>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
+    case 3: symbolic_g<64>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
+    case 4: symbolic_g<128>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
+    case 5: symbolic_g<256>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
+    case 6: symbolic_g<512>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
+    case 7: symbolic_g<1024>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
     case BIN_GLOBAL: {
         const int blocks = count < gws_blocks ? count : gws_blocks;
         hipLaunchKernelGGL(k_symbolic_global, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, rnnz,
@@ -588,8 +721,8 @@ hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int c
 
 template <int G, typename P>
 static void numeric_g(const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
-                      const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int *err, int num_cus,
-                      unsigned *queue, hipStream_t s)
+                      const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int two_level, int *err,
+                      int num_cus, unsigned *queue, hipStream_t s)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
@@ -598,24 +731,25 @@ static void numeric_g(const int *rows, const int *count_dev, int count, int max_
     const int64_t blocks = row_grid(k_numeric_lds<G, P>, BS, lds, ((int64_t)count + RPB - 1) / RPB, num_cus, queue);
     const int lg = __builtin_ctz(G);
     hipLaunchKernelGGL((k_numeric_lds<G, P>), dim3((unsigned)blocks), dim3(BS), lds, s, rows, count_dev, slice, op, crp,
-                       Ccol, (typename P::V *)Cval, bshift > lg ? lg : bshift, tshift > lg ? lg : tshift, err, queue);
+                       Ccol, (typename P::V *)Cval, bshift > lg ? lg : bshift, tshift > lg ? lg : tshift, two_level, err,
+                       queue);
 }
 
 template <typename P>
 static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
-                            const long long *crp, int *Ccol, void *Cval, float *comp, int bshift, int tshift, u64 *gws,
-                            long long gws_stride, long long gws_tiles, int gws_blocks, int *err, int num_cus,
-                            unsigned *queue, hipStream_t s)
+                            const long long *crp, int *Ccol, void *Cval, float *comp, int bshift, int tshift,
+                            int two_level, u64 *gws, long long gws_stride, long long gws_tiles, int gws_blocks, int *err,
+                            int num_cus, unsigned *queue, hipStream_t s)
 {
     switch (bin) {
-    case 0: numeric_g<8, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
-    case 1: numeric_g<16, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
-    case 2: numeric_g<32, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
-    case 3: numeric_g<64, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
-    case 4: numeric_g<128, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
-    case 5: numeric_g<256, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
-    case 6: numeric_g<512, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
-    case 7: numeric_g<1024, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, err, num_cus, queue, s); break;
+    case 0: numeric_g<8, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
+    case 1: numeric_g<16, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
+    case 2: numeric_g<32, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
+    case 3: numeric_g<64, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
+    case 4: numeric_g<128, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
+    case 5: numeric_g<256, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
+    case 6: numeric_g<512, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
+    case 7: numeric_g<1024, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
     case BIN_GLOBAL: {
         const int blocks = count < gws_blocks ? count : gws_blocks;
         hipLaunchKernelGGL(k_numeric_global<P>, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, crp, Ccol,
@@ -630,16 +764,16 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
 
 hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count, int max_need,
                           const Operands &op, const long long *crp, int *Ccol, void *Cval, float *comp, int bshift,
-                          int tshift, u64 *gws, long long gws_stride, long long gws_tiles, int gws_blocks, int *err,
-                          int num_cus, unsigned *queue, hipStream_t s)
+                          int tshift, int two_level, u64 *gws, long long gws_stride, long long gws_tiles,
+                          int gws_blocks, int *err, int num_cus, unsigned *queue, hipStream_t s)
 {
     if (count <= 0)
         return hipSuccess;
     if (fp32c)
         return numeric_p<PrecF32C>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, comp, bshift, tshift,
-                                   gws, gws_stride, gws_tiles, gws_blocks, err, num_cus, queue, s);
-    return numeric_p<PrecF64>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, nullptr, bshift, tshift, gws,
-                              gws_stride, gws_tiles, gws_blocks, err, num_cus, queue, s);
+                                   two_level, gws, gws_stride, gws_tiles, gws_blocks, err, num_cus, queue, s);
+    return numeric_p<PrecF64>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, nullptr, bshift, tshift,
+                              two_level, gws, gws_stride, gws_tiles, gws_blocks, err, num_cus, queue, s);
 }
 
 hipError_t configure_kernels(int lds_max)

@@ -638,7 +638,10 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     bp.lds_row_max = opts.lds_row_max > 0 ? opts.lds_row_max : lds_dyn;
     if (bp.lds_row_max > lds_dyn)
         bp.lds_row_max = lds_dyn;
-    bp.lds_block_budget = opts.lds_block_budget > 0 ? opts.lds_block_budget : 64 * 1024;
+    // 8 KiB per one-wave workgroup that packs several rows: 20 such workgroups fit a CU's
+    // 160 KiB; rows needing more move up to a bin with fewer rows per workgroup (measured
+    // best on the batched Laplacians)
+    bp.lds_block_budget = opts.lds_block_budget > 0 ? opts.lds_block_budget : 8 * 1024;
     if (bp.lds_block_budget > lds_dyn)
         bp.lds_block_budget = lds_dyn;
     bp.per_lane = opts.tiles_per_lane > 0 ? opts.tiles_per_lane : 4;
@@ -697,6 +700,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     op.tmask = tmask;
     op.rmin = rmin;
     op.rmax = rmax;
+    op.rtp = rtp;
     // Row directories (tile masks + slot offsets) of LDS-bin rows are kept from the
     // symbolic to the numeric phase when they fit the budget (12 bytes per spanned tile:
     // 15 MB on the banded workload); otherwise the numeric kernel rebuilds them.
@@ -725,7 +729,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
                 return rc;
         }
         HIP_TRY(launch_symbolic(b, binrows + (size_t)b * M, &dst->sym.count[b], cnt, h1.sym.max_need[b], op, rnnz,
-                                tshift, gws, gws_stride, c->num_cus, gws_blocks,
+                                tshift, h1.sym.nsparse[b] > 0 ? 1 : 0, gws, gws_stride, c->num_cus, gws_blocks,
                                 opts.persistent ? dst->queue[0][b] : nullptr, s));
     }
     // Arrays of C handed to the caller; dropped again (one-shot calls) on any failure.
@@ -807,8 +811,9 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][0], s));
         HIP_TRY(launch_numeric(b, fp32 ? 1 : 0, binrows + (size_t)b * M, &dst->num.count[b], cnt, h2.num.max_need[b], op,
-                               crp, Ccol, Cval, comp, bshift, tshift, gws, gws_stride, gws_tiles, gws_blocks, errp,
-                               c->num_cus, opts.persistent ? dst->queue[1][b] : nullptr, s));
+                               crp, Ccol, Cval, comp, bshift, tshift, h2.num.nsparse[b] > 0 ? 1 : 0, gws, gws_stride,
+                               gws_tiles, gws_blocks, errp, c->num_cus, opts.persistent ? dst->queue[1][b] : nullptr,
+                               s));
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][1], s));
     }

@@ -19,6 +19,37 @@ DEVI int mbcnt(u64 m)
 
 DEVI u64 below(int b) { return (1ull << b) - 1ull; }
 
+// Reductions and scans over the 64 lanes of a wave (every lane must take part).
+template <typename T> DEVI T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T> DEVI T wave_max(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+template <typename T> DEVI T wave_incl_scan(T v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const T u = __shfl_up(v, o, 64);
+        if (lane >= o)
+            v += u;
+    }
+    return v;
+}
+
 // Map the hardware block index so that each XCD (blocks are dealt round-robin over the
 // 8 XCDs) works on a contiguous range of logical blocks: neighbouring rows, which share
 // rows of B, then share one XCD's L2.
@@ -29,34 +60,73 @@ DEVI int xcd_block(int b, int n)
 }
 
 // Device work queue of the persistent row kernels: the n row groups of a launch are cut
-// into 8 contiguous ranges, one per XCD, each with a counter q[x] (zero at launch).  A
-// workgroup takes the next group of its own XCD's range (xq starts at blockIdx.x & 7) and,
-// once that range is used up, of the following ranges in turn, so the groups one XCD works
-// on stay neighbours (shared rows of B in its L2) until the queues drain.  Returns n when
-// no group is left.  Thread 0 pulls, the result goes to every thread through *s; the
-// caller's loop body must end with a workgroup barrier before the next pull.
-DEVI int pull_group(unsigned *q, int n, int &xq, int *s)
+// into 8 contiguous ranges, one per XCD, each with a counter q[x] (zero at launch) that
+// hands out chunks of `chunk` consecutive groups.  A workgroup takes the next chunk of its
+// own XCD's range (xq starts at blockIdx.x & 7) and, once that range is used up, of the
+// following ranges in turn, so the groups one XCD works on stay neighbours (shared rows
+// of B in its L2) until the queues drain.  Thread 0 pulls; the chunk [s[0], s[1]) goes to
+// every thread through LDS, s[0] = n when nothing is left.  The caller's loop body must
+// end with a workgroup barrier before the next pull.
+DEVI void pull_chunk(unsigned *q, int n, int chunk, int &xq, int *s)
 {
     if (threadIdx.x == 0) {
-        int got = n;
+        int got = n, end = n;
         for (int k = 0; k < 8; ++k) {
             const int x = (xq + k) & 7;
             const int lo = (int)(((long long)n * x) >> 3), hi = (int)(((long long)n * (x + 1)) >> 3);
-            const unsigned len = (unsigned)(hi - lo);
-            if (__hip_atomic_load(&q[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len)
+            const unsigned nch = (unsigned)((hi - lo + chunk - 1) / chunk);
+            if (__hip_atomic_load(&q[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= nch)
                 continue;
             const unsigned v = atomicAdd(&q[x], 1u);
-            if (v < len) {
-                got = lo + (int)v;
+            if (v < nch) {
+                got = lo + (int)v * chunk;
+                end = got + chunk < hi ? got + chunk : hi;
                 xq = x;
                 break;
             }
         }
-        *s = got;
+        s[0] = got;
+        s[1] = end;
     }
     __syncthreads();
-    return *s;
 }
+
+// The row groups a workgroup of a row kernel works on: with a queue, chunks pulled from
+// it; without, groups xcd_block(blockIdx) + k * gridDim (one group at a time).
+struct GroupWalk {
+    unsigned *q;
+    int n, chunk, xq, grp, end;
+    int *s; // two ints of LDS
+
+    DEVI GroupWalk(unsigned *queue, int ngroups, int *lds2) : q(queue), n(ngroups), xq(blockIdx.x & 7), s(lds2)
+    {
+        // chunks of about an eighth of a workgroup's share of its XCD's range
+        const long long per = (long long)ngroups / ((long long)gridDim.x * 8);
+        chunk = per > 1 ? (int)per : 1;
+        if (q) {
+            pull_chunk(q, n, chunk, xq, s);
+            grp = s[0];
+            end = s[1];
+        } else {
+            grp = xcd_block(blockIdx.x, gridDim.x);
+            end = grp + 1;
+        }
+    }
+    DEVI bool more() const { return grp < n; }
+    DEVI void next()
+    {
+        if (++grp < end)
+            return;
+        if (q) {
+            pull_chunk(q, n, chunk, xq, s);
+            grp = s[0];
+            end = s[1];
+        } else {
+            grp += (int)gridDim.x - 1;
+            end = grp + 1;
+        }
+    }
+};
 
 // Sum over a group of G consecutive lanes.  G <= 64: shuffles; G > 64: through LDS
 // (every thread of the block must call it).

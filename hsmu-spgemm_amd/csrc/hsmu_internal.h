@@ -10,12 +10,16 @@
 namespace hsmu {
 
 // Row bins.  Bins 0..7 keep a row in LDS with G = 8 << b lanes per row (rows per
-// workgroup = max(1, 256 / G)).  BIN_GLOBAL keeps the row's tile directory in global
+// workgroup = max(1, BLOCK / G)).  BIN_GLOBAL keeps the row's tile directory in global
 // memory and accumulates straight into C (rows whose LDS need exceeds the limit).
 constexpr int NLDS = 8;
 constexpr int BIN_GLOBAL = 8;
 constexpr int NBINS = 10;  // slot 9 reserved
-constexpr int BLOCK = 256; // workgroup size of the multi-row bins
+// Workgroup size of the bins that pack several rows per workgroup (G < 64 lanes per
+// row): one wave, so the phases of a row group are separated by single-wave barriers and
+// every wave runs at its own pace (measured: 4-wave workgroups of 8-lane rows spend most
+// of their time waiting at barriers for the slowest wave).
+constexpr int BLOCK = 64;
 
 // error bits raised on the device, reported at the next host synchronisation
 constexpr int ERR_ROWPTR = 1;   // row pointer decreasing or out of [0, nnz]
@@ -40,6 +44,7 @@ struct BinStats {
     unsigned long long work[NBINS]; // products (numeric) or tile-products (symbolic)
     unsigned long long nnz_c[NBINS];// nnz of C in the bin's rows (numeric)
     unsigned long long nnz_a[NBINS];// nnz of A in the bin's rows
+    int nsparse[NBINS];             // rows with a two-level tile directory
 };
 
 // Device-side counters of one call, copied to pinned host memory at the two
@@ -88,10 +93,32 @@ struct BatchOffsets {
 // one and saves the two directory reads per product) and at most 128 tiles wide.
 HSMU_HD inline bool use_dense(long long S, long long n) { return S <= 128 && 2 * 64 * S <= 3 * n; }
 
+// Tile directory of a row of C over its span [rmin, rmax] of S tiles.
+// Flat: one entry (mask, first slot) per tile of the span, tile t at index t.
+// Two-level, for rows whose span is mostly empty (the 3D stencil's A^2 row touches ~10 of
+// ~140 tiles): a coarse bitmap of W = ceil(S/64) words marks the non-empty tiles, with
+// coff[w] the compact index of the first non-empty tile of word w; the fine level holds
+// only the non-empty tiles, tile t at compact index coff[t>>6] + popcount(cm[t>>6] &
+// below(t&63)) -- the HSMU rank trick applied to tiles as well as to columns, so a slot
+// is still found in O(1).  At most R = min(S, tile-products) fine entries.  Chosen when
+// it at least halves the directory (never for spans of up to 64 tiles).
+HSMU_HD inline long long fine_cap(long long S, long long tp) { return tp < S ? tp : S; }
+HSMU_HD inline bool use_sparse(long long S, long long tp)
+{
+    return S > 64 && 2 * ((S + 63) / 64 + fine_cap(S, tp)) <= S;
+}
+// directory entries of a row (12 bytes each when kept between the phases)
+HSMU_HD inline long long dir_entries(long long S, long long tp)
+{
+    return use_sparse(S, tp) ? (S + 63) / 64 + fine_cap(S, tp) : S;
+}
+
 // LDS slice of one row in the LDS kernels (byte offsets), for a row whose C row spans
-// S tiles, has n = nnz(C_i) entries and nA entries in A:
-//   dm   u64[S]     tile masks of the row of C over its span [rmin, rmax]
-//   doff u32[S]     numeric: slot of each tile's first column
+// S tiles, has n = nnz(C_i) entries, nA entries in A and tp tile-products; E =
+// dir_entries(S, tp):
+//   dm   u64[E]     tile masks (flat), or coarse words then fine masks (two-level)
+//   doff u32[E]     slot of each tile's first column (numeric), coarse offsets (two-level)
+//   ft   u32[R]     numeric, two-level: tile of each fine entry, relative to rmin
 //   vals f64[n]     numeric: accumulators (64*S of them in the dense form)
 //   av   f64[nA]    numeric: a_ij of the row's entries
 //   tlen i32[nA]    tiles(B_j) for each entry j of the row
@@ -100,22 +127,27 @@ HSMU_HD inline bool use_dense(long long S, long long n) { return S <= 128 && 2 *
 //   b0   i32[nA]    numeric: first stored element of B_j
 // The size is rounded up to 16 bytes so per-row slices of a workgroup stay aligned.
 struct RowLayout {
-    long long dm, doff, vals, av, tlen, t0, blen, b0, bytes;
+    long long dm, doff, ft, vals, av, tlen, t0, blen, b0, bytes;
 };
 
-HSMU_HD inline RowLayout row_layout(int numeric, long long S, long long n, long long nA)
+HSMU_HD inline RowLayout row_layout(int numeric, long long S, long long n, long long nA, long long tp)
 {
     RowLayout L;
+    const bool sparse = use_sparse(S, tp);
+    const long long E = dir_entries(S, tp);
     long long o = 0;
     L.dm = o;
-    o += 8 * S;
+    o += 8 * E;
     L.doff = o;
-    if (numeric)
-        o += 4 * S;
+    if (numeric || sparse)
+        o += 4 * E;
+    L.ft = o;
+    if (numeric && sparse)
+        o += 4 * fine_cap(S, tp);
     o = (o + 7) & ~7LL;
     L.vals = o;
     if (numeric)
-        o += 8 * (use_dense(S, n) ? 64 * S : n);
+        o += 8 * (!sparse && use_dense(S, n) ? 64 * S : n);
     L.av = o;
     if (numeric)
         o += 8 * nA;
@@ -133,8 +165,11 @@ HSMU_HD inline RowLayout row_layout(int numeric, long long S, long long n, long 
     return L;
 }
 
-HSMU_HD inline long long sym_need(long long S, long long nA) { return row_layout(0, S, 0, nA).bytes; }
-HSMU_HD inline long long num_need(long long S, long long n, long long nA) { return row_layout(1, S, n, nA).bytes; }
+HSMU_HD inline long long sym_need(long long S, long long nA, long long tp) { return row_layout(0, S, 0, nA, tp).bytes; }
+HSMU_HD inline long long num_need(long long S, long long n, long long nA, long long tp)
+{
+    return row_layout(1, S, n, nA, tp).bytes;
+}
 
 // LDS kept back from the dynamic allocation for the kernels' static __shared__ arrays
 constexpr int LDS_STATIC_RESERVE = 256;
@@ -175,8 +210,11 @@ struct Operands {
     const int *tptr, *tcol;
     const unsigned long long *tmask;
     const int *rmin, *rmax;
+    const int *rtp; // tile-products per row (chooses the directory form)
     // Row directories kept between the phases (nullptr when not kept): the symbolic LDS
-    // kernel stores row i's tile masks and slot offsets at dir_mask/dir_off + rdir[i]
+    // kernel stores row i's directory (flat: tile masks and slot offsets; two-level: the
+    // coarse words and compact indices, then the fine masks and slot offsets) at
+    // dir_mask/dir_off + rdir[i]
     // (rdir < 0: not stored), the numeric kernel loads them instead of rebuilding them.
     const long long *rdir;
     unsigned long long *dir_mask;
@@ -188,8 +226,9 @@ struct Operands {
 // gws_tiles >= the widest tile span in the bin.
 // queue: nullptr = one workgroup per row group (grid from count); otherwise 8 zeroed
 // counters and the LDS kernels run persistent (grid from occupancy x num_cus).
+// two_level: some row of the bin has a two-level directory (BinStats::nsparse > 0).
 hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int count,
-                           int max_need, const Operands &op, int *rnnz, int tshift,
+                           int max_need, const Operands &op, int *rnnz, int tshift, int two_level,
                            unsigned long long *gws, long long gws_stride, int num_cus,
                            int gws_blocks, unsigned *queue, hipStream_t s);
 
@@ -200,9 +239,9 @@ hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *s
 // FP32 (Cval float; comp = nnz(C) floats of scratch, needed only for BIN_GLOBAL).
 hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count,
                           int max_need, const Operands &op, const long long *crp, int *Ccol,
-                          void *Cval, float *comp, int bshift, int tshift, unsigned long long *gws,
-                          long long gws_stride, long long gws_tiles, int gws_blocks, int *err,
-                          int num_cus, unsigned *queue, hipStream_t s);
+                          void *Cval, float *comp, int bshift, int tshift, int two_level,
+                          unsigned long long *gws, long long gws_stride, long long gws_tiles,
+                          int gws_blocks, int *err, int num_cus, unsigned *queue, hipStream_t s);
 
 hipError_t launch_rowptr_i64_to_i32(const long long *in, int64_t n, int *out, hipStream_t s);
 

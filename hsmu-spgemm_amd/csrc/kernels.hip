@@ -479,10 +479,11 @@ DEVI int choose_bin(long long work, long long need, const BinParams &bp)
     int b = 0;
     while (b < NLDS - 1 && (8LL << b) < lanes)
         ++b;
+    // the budget limits only workgroups that pack several rows: a one-row workgroup needs
+    // the row's LDS whatever its width
     while (b < NLDS - 1) {
         const int G = 8 << b;
-        const long long rpb = G >= BLOCK ? 1 : BLOCK / G;
-        if (need * rpb <= bp.lds_block_budget)
+        if (G >= BLOCK || need * (BLOCK / G) <= bp.lds_block_budget)
             break;
         ++b;
     }
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
                                               const int *__restrict__ rnnz, BinParams bp, int *__restrict__ binrows,
                                               long long *__restrict__ rdir, DevStats *st)
 {
-    __shared__ int s_cnt[NBINS], s_need[NBINS], s_span[NBINS], s_base[NBINS];
+    __shared__ int s_cnt[NBINS], s_need[NBINS], s_span[NBINS], s_base[NBINS], s_sparse[NBINS];
     __shared__ unsigned long long s_work[NBINS], s_nc[NBINS], s_na[NBINS];
     __shared__ unsigned long long s_dir, s_dir_base;
     if (threadIdx.x < NBINS) {
@@ -508,35 +509,73 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
         s_work[threadIdx.x] = 0;
         s_nc[threadIdx.x] = 0;
         s_na[threadIdx.x] = 0;
+        s_sparse[threadIdx.x] = 0;
     }
     if (threadIdx.x == 0)
         s_dir = 0;
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
-    int b = -1, pos = 0;
-    long long dloc = -1;
+    int b = -1, S = 0, nA = 0, nc = 0, sp = 0;
+    long long work = 0, need = 0, E = 0;
     if (i < M && rprod[i] > 0) {
-        const int S = rmax[i] - rmin[i] + 1;
-        const int nA = Arp[i + 1] - Arp[i];
-        long long work, need;
-        int nc = 0;
+        S = rmax[i] - rmin[i] + 1;
+        nA = Arp[i + 1] - Arp[i];
+        const int tp = rtp[i];
         if (numeric) {
             nc = rnnz[i];
-            need = num_need(S, nc, nA);
+            need = num_need(S, nc, nA, tp);
             work = rprod[i];
         } else {
-            need = sym_need(S, nA);
-            work = rtp[i];
+            need = sym_need(S, nA, tp);
+            work = tp;
         }
         b = choose_bin(work, need, bp);
-        pos = atomicAdd(&s_cnt[b], 1);
-        atomicMax(&s_need[b], (int)(need < INT_MAX ? need : INT_MAX));
-        atomicMax(&s_span[b], S);
-        atomicAdd(&s_work[b], (unsigned long long)work);
-        atomicAdd(&s_nc[b], (unsigned long long)nc);
-        atomicAdd(&s_na[b], (unsigned long long)nA);
+        sp = use_sparse(S, tp) ? 1 : 0;
         if (!numeric && b < BIN_GLOBAL)
-            dloc = (long long)atomicAdd(&s_dir, (unsigned long long)S);
+            E = dir_entries(S, tp);
+    }
+    // Aggregate per wave first: the lanes that chose one bin (neighbouring rows usually
+    // share it) add their counts and sums with one LDS atomic per bin per wave, instead
+    // of one per lane on the same LDS word.
+    const int lane = threadIdx.x & 63;
+    int pos = 0;
+    long long dloc = -1;
+    u64 pending = __ballot(b >= 0);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int bb = __shfl(b, leader, 64);
+        const u64 same = __ballot(b == bb);
+        pending &= ~same;
+        const bool mine = b == bb;
+        const unsigned long long wsum = wave_sum<unsigned long long>(mine ? (unsigned long long)work : 0ull);
+        const unsigned long long ncs = wave_sum<unsigned long long>(mine ? (unsigned long long)nc : 0ull);
+        const unsigned long long nas = wave_sum<unsigned long long>(mine ? (unsigned long long)nA : 0ull);
+        const int needm = wave_max<int>(mine ? (int)(need < INT_MAX ? need : INT_MAX) : 0);
+        const int spanm = wave_max<int>(mine ? S : 0);
+        const int sps = wave_sum<int>(mine ? sp : 0);
+        const unsigned long long dincl = wave_incl_scan<unsigned long long>(mine ? (unsigned long long)E : 0ull);
+        const unsigned long long dtot = __shfl(dincl, 63, 64);
+        int base = 0;
+        unsigned long long dbase = 0;
+        if (lane == leader) {
+            base = atomicAdd(&s_cnt[bb], __popcll(same));
+            atomicMax(&s_need[bb], needm);
+            atomicMax(&s_span[bb], spanm);
+            atomicAdd(&s_work[bb], wsum);
+            atomicAdd(&s_nc[bb], ncs);
+            atomicAdd(&s_na[bb], nas);
+            if (sps)
+                atomicAdd(&s_sparse[bb], sps);
+            if (dtot)
+                dbase = atomicAdd(&s_dir, dtot);
+        }
+        base = __shfl(base, leader, 64);
+        dbase = __shfl(dbase, leader, 64);
+        if (mine) {
+            pos = base + mbcnt(same);
+            if (E > 0)
+                dloc = (long long)(dbase + dincl) - E;
+        }
     }
     __syncthreads();
     BinStats *bs = numeric ? &st->num : &st->sym;
@@ -549,6 +588,8 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
             atomicAdd(&bs->work[threadIdx.x], s_work[threadIdx.x]);
             atomicAdd(&bs->nnz_c[threadIdx.x], s_nc[threadIdx.x]);
             atomicAdd(&bs->nnz_a[threadIdx.x], s_na[threadIdx.x]);
+            if (s_sparse[threadIdx.x])
+                atomicAdd(&bs->nsparse[threadIdx.x], s_sparse[threadIdx.x]);
         }
     }
     if (!numeric && threadIdx.x == NBINS)
@@ -634,32 +675,48 @@ __global__ __launch_bounds__(256) void k_batch_rowptr(int which, int count, cons
     }
 }
 
+// Entries are stacked in chunks of STACK_CHUNK per workgroup: one binary search per chunk
+// for its first instance, then each thread steps forward from there (a chunk spans few
+// instances), so the lookup costs no chain of dependent loads per entry.
+constexpr int STACK_CHUNK = 4096;
+
 __global__ __launch_bounds__(256) void k_batch_entries(int which, int count, const BatchInst *__restrict__ inst,
                                                        const long long *__restrict__ eoff,
                                                        const long long *__restrict__ koff, int64_t nnz, int val32,
                                                        int *__restrict__ col, void *__restrict__ val, DevStats *st)
 {
-    const int64_t stride = (int64_t)gridDim.x * 256;
+    __shared__ int s_i0;
     int bad = 0;
-    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nnz; e += stride) {
-        const int i = find_instance(eoff, count, e, 0);
-        const BatchInst &I = inst[i];
-        const long long k = e - eoff[i];
-        const int c = which ? I.Bcol[k] : I.Acol[k];
-        const int lim = which ? I.N : I.K;
-        int cs = c;
-        if (c < 0 || c >= lim) {
-            bad |= which ? ERR_BCOL : ERR_ACOL;
-            cs = 0;
+    for (int64_t base = (int64_t)blockIdx.x * STACK_CHUNK; base < nnz; base += (int64_t)gridDim.x * STACK_CHUNK) {
+        if (threadIdx.x == 0)
+            s_i0 = find_instance(eoff, count, base, 0);
+        __syncthreads();
+        int i = s_i0;
+        for (int k = threadIdx.x; k < STACK_CHUNK; k += 256) {
+            const int64_t e = base + k;
+            if (e >= nnz)
+                break;
+            while (eoff[i + 1] <= e) // ends: e < nnz = eoff[count]
+                ++i;
+            const BatchInst &I = inst[i];
+            const long long q = e - eoff[i];
+            const int c = which ? I.Bcol[q] : I.Acol[q];
+            const int lim = which ? I.N : I.K;
+            int cs = c;
+            if (c < 0 || c >= lim) {
+                bad |= which ? ERR_BCOL : ERR_ACOL;
+                cs = 0;
+            }
+            col[e] = which ? cs : (int)(cs + koff[i]);
+            if (val) {
+                const void *src = which ? I.Bval : I.Aval;
+                if (val32)
+                    ((float *)val)[e] = ((const float *)src)[q];
+                else
+                    ((double *)val)[e] = ((const double *)src)[q];
+            }
         }
-        col[e] = which ? cs : (int)(cs + koff[i]);
-        if (val) {
-            const void *src = which ? I.Bval : I.Aval;
-            if (val32)
-                ((float *)val)[e] = ((const float *)src)[k];
-            else
-                ((double *)val)[e] = ((const double *)src)[k];
-        }
+        __syncthreads(); // s_i0 is rewritten for the next chunk
     }
     if (bad)
         atomicOr(&st->err, bad);
@@ -706,7 +763,10 @@ hipError_t launch_batch_entries(int which, int64_t count, const BatchInst *inst,
 {
     if (nnz <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_batch_entries, dim3(grid_for(nnz)), dim3(256), 0, s, which, (int)count, inst,
+    int64_t chunks = (nnz + STACK_CHUNK - 1) / STACK_CHUNK;
+    if (chunks > 16384)
+        chunks = 16384;
+    hipLaunchKernelGGL(k_batch_entries, dim3((unsigned)chunks), dim3(256), 0, s, which, (int)count, inst,
                        which ? off.eb : off.ea, off.ko, nnz, val32, col, val, st);
     return hipGetLastError();
 }
@@ -728,7 +788,8 @@ __global__ void k_plan_check(DevStats *st, const DevStats *expect, int stage)
     const BinStats &b = stage == 0 ? expect->sym : expect->num;
     bool same = true;
     for (int k = 0; k < NBINS; ++k)
-        same = same && a.count[k] == b.count[k] && a.max_need[k] == b.max_need[k] && a.max_span[k] == b.max_span[k];
+        same = same && a.count[k] == b.count[k] && a.max_need[k] == b.max_need[k] && a.max_span[k] == b.max_span[k] &&
+               a.nsparse[k] == b.nsparse[k];
     if (stage == 0)
         same = same && st->dir_total == expect->dir_total && st->ntiles_b == expect->ntiles_b;
     else

@@ -6,8 +6,8 @@
 // per shape class of the input by timing candidates on the device, keeping the winners
 // in a plain-text table:
 //
-//   # class products_per_lane tiles_per_lane b_lanes t_lanes lds_block_budget persistent ms
-//   c10_a3_b3_m22 32 4 0 0 65536 1 3.512
+//   # class products_per_lane tiles_per_lane b_lanes t_lanes lds_block_budget persistent rebuild_dirs ms
+//   c10_a3_b3_m22 32 4 0 0 65536 1 0 3.512
 //
 // Lines are appended; the last line of a class wins.  Tuning changes only how rows are
 // grouped and walked, never the result's structure (FP64 values may differ in the last
@@ -26,12 +26,13 @@
 namespace {
 
 struct Knobs {
-    int ppl, tpl, bl, tl, budget, persistent;
+    int ppl, tpl, bl, tl, budget, persistent, rebuild;
 };
 
 Knobs from_opts(const hsmu_opts &o)
 {
-    return {o.products_per_lane, o.tiles_per_lane, o.b_lanes, o.t_lanes, o.lds_block_budget, o.persistent};
+    return {o.products_per_lane, o.tiles_per_lane, o.b_lanes, o.t_lanes, o.lds_block_budget, o.persistent,
+            o.rebuild_dirs};
 }
 
 void to_opts(const Knobs &k, hsmu_opts &o)
@@ -42,6 +43,7 @@ void to_opts(const Knobs &k, hsmu_opts &o)
     o.t_lanes = k.tl;
     o.lds_block_budget = k.budget;
     o.persistent = k.persistent;
+    o.rebuild_dirs = k.rebuild;
 }
 
 bool lookup(const char *path, const std::string &cls, Knobs &k)
@@ -59,8 +61,8 @@ bool lookup(const char *path, const std::string &cls, Knobs &k)
         char name[128];
         Knobs t;
         double ms;
-        if (sscanf(line, "%127s %d %d %d %d %d %d %lf", name, &t.ppl, &t.tpl, &t.bl, &t.tl, &t.budget, &t.persistent,
-                   &ms) == 8 &&
+        if (sscanf(line, "%127s %d %d %d %d %d %d %d %lf", name, &t.ppl, &t.tpl, &t.bl, &t.tl, &t.budget,
+                   &t.persistent, &t.rebuild, &ms) == 9 &&
             cls == name) {
             k = t;
             found = true;
@@ -164,7 +166,8 @@ extern "C" int hsmu_autotune(hsmu_ctx *ctx, int64_t count, const hsmu_csr *A, co
         {&Knobs::bl, {0, 2, 4, 8, 16, 32}},
         {&Knobs::tpl, {2, 4, 8, 16}},
         {&Knobs::tl, {0, 1, 2, 4, 8}},
-        {&Knobs::budget, {32768, 65536, 98304, 131072}},
+        {&Knobs::budget, {2048, 4096, 6144, 8192, 12288, 16384, 32768}},
+        {&Knobs::rebuild, {0, 1}},
     };
     for (const Axis &ax : axes) {
         const int keep = k.*(ax.field);
@@ -195,7 +198,8 @@ extern "C" int hsmu_autotune(hsmu_ctx *ctx, int64_t count, const hsmu_csr *A, co
     if (table_path) {
         FILE *f = fopen(table_path, "a");
         if (f) {
-            fprintf(f, "%s %d %d %d %d %d %d %.4f\n", cls, k.ppl, k.tpl, k.bl, k.tl, k.budget, k.persistent, best);
+            fprintf(f, "%s %d %d %d %d %d %d %d %.4f\n", cls, k.ppl, k.tpl, k.bl, k.tl, k.budget, k.persistent,
+                    k.rebuild, best);
             fclose(f);
         }
     }

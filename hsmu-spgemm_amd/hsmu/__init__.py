@@ -198,6 +198,17 @@ def gen_zipf(M: int = 500000, N: int = 100000, per_row: int = 8, s: float = 1.2,
     return _gen(lib.hsmu_gen_zipf, M, N, per_row, s, seed)
 
 
+def gen_laplace_batch(count: int = 512, seed: int = 3) -> list[HostMatrix]:
+    """The batched workload: count // 2 2D 5-point Laplacians with side
+    floor(exp(U(ln 16, ln 256))), then count - count // 2 3D 7-point Laplacians with side
+    floor(exp(U(ln 8, ln 48))), sides drawn with numpy's default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    n2 = count // 2
+    s2 = np.floor(np.exp(rng.uniform(np.log(16), np.log(256), n2))).astype(int)
+    s3 = np.floor(np.exp(rng.uniform(np.log(8), np.log(48), count - n2))).astype(int)
+    return [gen_laplace2d(int(s)) for s in s2] + [gen_laplace3d(int(s)) for s in s3]
+
+
 def gen_random(M: int, N: int, max_per_row: int, seed: int) -> HostMatrix:
     return _gen(lib.hsmu_gen_random, M, N, max_per_row, seed)
 

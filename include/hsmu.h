@@ -110,7 +110,7 @@ typedef struct hsmu_opts {
     int32_t lds_row_max;      /* bytes of LDS one row may use before it falls back to the
                                  global-memory path; 0 = device maximum */
     int32_t lds_block_budget; /* LDS bytes per workgroup for bins that pack several rows
-                                 into one workgroup; 0 = default (64 KiB) */
+                                 into one workgroup (one wave); 0 = default (8 KiB) */
     int32_t products_per_lane;/* numeric work per lane that sets a row's lane count; 0 = 32 */
     int32_t tiles_per_lane;   /* symbolic work per lane that sets a row's lane count; 0 = 4 */
     int32_t timing;           /* 1: record per-stage hipEvents into hsmu_stats */
@@ -222,11 +222,12 @@ void hsmu_plan_destroy(hsmu_plan *plan); /* releases the plan's buffers, C[i] in
 
 /* ---- autotuner ------------------------------------------------------------------------
  * Bin thresholds (products_per_lane, tiles_per_lane), sub-group widths (b_lanes,
- * t_lanes), the LDS budget per workgroup and the persistent mode, chosen per SHAPE CLASS
+ * t_lanes), the LDS budget per workgroup, the persistent mode and whether the numeric
+ * phase rebuilds the tile directories (rebuild_dirs), chosen per SHAPE CLASS
  * of the input: the class is "c<C>_a<Ea>_b<Eb>_m<Em>" from the instance count bucket C
  * (1, or log2 of the count), the rounded log2 of the mean row length of A and of B, and
  * the floor of log2 of the total rows of A (hsmu_shape_class).  The table is a text
- * file, one line per class: class, the six values, the measured ms.  When the class is
+ * file, one line per class: class, the seven values, the measured ms.  When the class is
  * in the table its values are returned; otherwise, if search is nonzero, the candidates
  * are timed on this input (coordinate descent from *base, each candidate run and timed
  * three times, best median kept), the winner is appended to the table (if table_path is

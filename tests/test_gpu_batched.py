@@ -325,7 +325,8 @@ def test_autotune_table_roundtrip(ctx, tmp_path):
         assert len(lines) == 1 and lines[0].split()[0] == cls
         o2, found2 = ctx.autotune(dA, dB, table=table, search=False)
         assert found2
-        for f in ("products_per_lane", "tiles_per_lane", "b_lanes", "t_lanes", "lds_block_budget", "persistent"):
+        for f in ("products_per_lane", "tiles_per_lane", "b_lanes", "t_lanes", "lds_block_budget", "persistent",
+                  "rebuild_dirs"):
             assert getattr(o1, f) == getattr(o2, f)
         cs, _ = ctx.spgemm_batched_device(dA, dB, o2)
         check_views(ctx, cs, pairs, exact=True)

@@ -51,7 +51,7 @@ def test_laplace2d_128(ctx):
     assert np.array_equal(Cm.val, ref.val)  # integer-valued: exact
 
 
-@pytest.mark.parametrize("n", [2, 9, 24])
+@pytest.mark.parametrize("n", [2, 9, 24, 40])
 def test_laplace3d(ctx, n):
     A = hsmu.gen_laplace3d(n)
     Cm, _, _ = check(ctx, A, A, f"laplace3d {n}")
