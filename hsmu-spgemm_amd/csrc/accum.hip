@@ -51,6 +51,12 @@ DEVI void stage_row(const Operands &op, int a0, int nA, int *t0, int *tlen, int 
 
 constexpr int UNROLL = 4;
 
+// A B row with at least LONG_SEG elements (left) is walked by a whole group of lanes --
+// the row's G lanes in the LDS kernels, a wave in the window kernels -- and shorter ones by
+// sub-groups: R-MAT's hub rows of B hold tens of thousands of entries and would otherwise
+// leave one sub-group walking while the rest of the workgroup waits.
+constexpr int LONG_SEG = 256;
+
 // Compact index of tile t (relative to rmin) in a two-level directory: the rank of t
 // among the row's non-empty tiles.
 DEVI unsigned fine_index(const u64 *cm, const unsigned *coff, int t)
@@ -137,8 +143,6 @@ struct PrecF64 {
     typedef double V;
     static DEVI void add(double *vals, unsigned slot, double a, V b) { atomicAdd(&vals[slot], a * b); }
     static DEVI V result(const double *vals, unsigned slot) { return vals[slot]; }
-    // global-memory accumulation (rows outside LDS): straight into C
-    static DEVI void gadd(V *Cval, float *, long long q, double a, V b) { unsafeAtomicAdd(&Cval[q], a * b); }
 };
 
 struct PrecF32C {
@@ -148,7 +152,7 @@ struct PrecF32C {
     // does not match, and the TwoSum error below must describe the addition actually
     // stored.  The error of that addition plus e then goes into c, whose own rounding is
     // second order (|c| ~ u |s|).
-    template <bool LDS> static DEVI void two_add(float *s, float *c, float p, float e)
+    static DEVI void two_add(float *s, float *c, float p, float e)
     {
         unsigned *si = (unsigned *)s;
         float old = *s;
@@ -163,10 +167,7 @@ struct PrecF32C {
         }
         const float bb = sum - old;
         const float err = (old - (sum - bb)) + (p - bb);
-        if constexpr (LDS)
-            atomicAdd(c, err + e);
-        else
-            unsafeAtomicAdd(c, err + e);
+        atomicAdd(c, err + e);
     }
     static DEVI void add(double *vals, unsigned slot, double a, V b)
     {
@@ -174,20 +175,12 @@ struct PrecF32C {
         const float pr = __fmul_rn(af, b);
         const float e = __builtin_fmaf(af, b, -pr);
         float *f = (float *)vals + 2 * slot;
-        two_add<true>(f, f + 1, pr, e);
+        two_add(f, f + 1, pr, e);
     }
     static DEVI V result(const double *vals, unsigned slot)
     {
         const float *f = (const float *)vals + 2 * slot;
         return f[0] + f[1];
-    }
-    // global-memory accumulation: s in C, c in a compensation array of nnz(C) floats
-    static DEVI void gadd(V *Cval, float *comp, long long q, double a, V b)
-    {
-        const float af = (float)a;
-        const float pr = __fmul_rn(af, b);
-        const float e = __builtin_fmaf(af, b, -pr);
-        two_add<false>(&Cval[q], &comp[q], pr, e);
     }
 };
 
@@ -199,15 +192,54 @@ enum { SLOT_FLAT, SLOT_DENSE, SLOT_TWO_LEVEL };
 // loads issued before the ds_add_f64s.  fm/foff: the (fine) tile masks and first slots;
 // cm/coff: the coarse level of a two-level directory.
 template <int G, int FORM, typename P>
-DEVI void add_products(const Operands &op, const int *b0, const int *blen, const double *av, int nA, const u64 *cm,
-                       const unsigned *coff, const u64 *fm, const unsigned *foff, double *vals, int mn, int bshift)
+DEVI void add_products(const Operands &op, const int *b0, const int *blen, const double *av, int nA, const int *lidx,
+                       int nlong, const u64 *cm, const unsigned *coff, const u64 *fm, const unsigned *foff, double *vals,
+                       int mn, int bshift)
 {
     typedef typename P::V V;
     const V *__restrict__ Bval = (const V *)op.Bval;
     const int l = threadIdx.x & (G - 1);
     const int SG = 1 << bshift, sl = l & (SG - 1), NSG = G >> bshift;
+    auto add = [&](int c, V v, double a) {
+        unsigned slot;
+        if constexpr (FORM == SLOT_DENSE) {
+            slot = (unsigned)(c - (mn << 6));
+        } else {
+            const int t = (c >> 6) - mn;
+            const unsigned k = FORM == SLOT_TWO_LEVEL ? fine_index(cm, coff, t) : (unsigned)t;
+            slot = foff[k] + (unsigned)__popcll(fm[k] & below(c & 63));
+        }
+        P::add(vals, slot, a, v);
+    };
+    // long B rows (listed in lidx): all G lanes of the row, UNROLL elements per lane per step
+    for (int i = 0; i < nlong; ++i) {
+        const int e = lidx[i];
+        const int kb = b0[e], ke = kb + blen[e];
+        const double a = av[e];
+        for (int k = kb + l; k < ke; k += UNROLL * G) {
+            int c[UNROLL];
+            V v[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                const int ku = k + u * G;
+                c[u] = -1;
+                v[u] = 0;
+                if (ku < ke) {
+                    c[u] = op.Bcol[ku];
+                    v[u] = Bval[ku];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+                if (c[u] >= 0)
+                    add(c[u], v[u], a);
+        }
+    }
+    // the others: sub-groups of SG lanes take one entry each
     for (int e = l >> bshift; e < nA; e += NSG) {
         const int kb = b0[e], ke = kb + blen[e];
+        if (ke - kb >= LONG_SEG)
+            continue;
         const double a = av[e];
         for (int k = kb + sl; k < ke; k += UNROLL * SG) {
             int c[UNROLL];
@@ -223,19 +255,9 @@ DEVI void add_products(const Operands &op, const int *b0, const int *blen, const
                 }
             }
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                if (c[u] >= 0) {
-                    unsigned slot;
-                    if constexpr (FORM == SLOT_DENSE) {
-                        slot = (unsigned)(c[u] - (mn << 6));
-                    } else {
-                        const int t = (c[u] >> 6) - mn;
-                        const unsigned k = FORM == SLOT_TWO_LEVEL ? fine_index(cm, coff, t) : (unsigned)t;
-                        slot = foff[k] + (unsigned)__popcll(fm[k] & below(c[u] & 63));
-                    }
-                    P::add(vals, slot, a, v[u]);
-                }
-            }
+            for (int u = 0; u < UNROLL; ++u)
+                if (c[u] >= 0)
+                    add(c[u], v[u], a);
         }
     }
 }
@@ -346,21 +368,24 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
     }
 }
 
-// Rows too wide for LDS: the same dense directory in a per-workgroup global workspace,
-// updated with L2 atomics.  One 1024-lane workgroup per row.
+// Rows too wide for LDS: the flat directory over the row's span in global memory, updated
+// with L2 atomics -- in the kept directory (rdir) when the directories are kept, so the
+// numeric phase reads masks and slot offsets from there, else in a per-workgroup
+// workspace.  One 1024-lane workgroup per row.
 __global__ __launch_bounds__(1024) void k_symbolic_global(const int *__restrict__ rows, const int *__restrict__ count_p,
                                                           Operands op, int *__restrict__ rnnz, int tshift, u64 *gws,
                                                           long long gws_stride)
 {
-    __shared__ int red[16];
+    __shared__ unsigned red[16];
     const int l = threadIdx.x;
-    u64 *dm = gws + (size_t)blockIdx.x * gws_stride;
     const int nrows = *count_p;
     const int SG = 1 << tshift, sl = l & (SG - 1), sg = l >> tshift, NSG = 1024 >> tshift;
     for (int idx = blockIdx.x; idx < nrows; idx += gridDim.x) {
         const int row = rows[idx];
         const int mn = op.rmin[row];
         const int S = op.rmax[row] - mn + 1;
+        const long long rd = op.dir_mask ? op.rdir[row] : -1;
+        u64 *dm = rd >= 0 ? op.dir_mask + rd : gws + (size_t)blockIdx.x * gws_stride;
         for (int t = l; t < S; t += 1024)
             dm[t] = 0;
         block_sync_global();
@@ -372,12 +397,17 @@ __global__ __launch_bounds__(1024) void k_symbolic_global(const int *__restrict_
                 atomicOr(&dm[op.tcol[t] - mn], op.tmask[t]);
         }
         block_sync_global();
-        int cnt = 0;
-        for (int t = l; t < S; t += 1024)
-            cnt += __popcll(dm[t]);
-        cnt = group_sum<1024>(cnt, red);
+        int ta, tb;
+        unsigned tot;
+        unsigned pre = scan_popcounts<1024>(dm, S, red, tot, ta, tb);
+        if (rd >= 0) {
+            for (int t = ta; t < tb; ++t) {
+                op.dir_off[rd + t] = pre;
+                pre += (unsigned)__popcll(dm[t]);
+            }
+        }
         if (l == 0)
-            rnnz[row] = cnt;
+            rnnz[row] = (int)tot;
         block_sync_global();
     }
 }
@@ -433,6 +463,8 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
         int *t0 = (int *)(slice + L.t0);
         int *blen = (int *)(slice + L.blen);
         int *b0 = (int *)(slice + L.b0);
+        int *lidx = (int *)(slice + L.lidx);
+        int *lcnt = (int *)(slice + L.lcnt);
         // fine level: the flat directory itself, or the compact entries after the coarse words
         u64 *fm = sparse ? dm + W : dm;
         unsigned *foff = sparse ? doff + W : doff;
@@ -441,6 +473,8 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
         const long long rd = (row >= 0 && op.dir_mask) ? op.rdir[row] : -1;
         for (int q = l; q < nv; q += G)
             vals[q] = 0.0;
+        if (l == 0)
+            *lcnt = 0;
         if (rd >= 0) {
             // directory kept by the symbolic phase: load its first level, stage only B's ranges
             for (int t = l; t < W; t += G) {
@@ -454,6 +488,10 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
             stage_row<G, true>(op, a0, nA, t0, tlen, b0, blen, av);
         }
         __syncthreads();
+        // entries with long rows of B, walked by the whole group
+        for (int k = l; k < nA; k += G)
+            if (blen[k] >= LONG_SEG)
+                lidx[atomicAdd(lcnt, 1)] = k;
         if (rd < 0) {
             if (sparse)
                 or_tiles<G, OR_COARSE>(op, t0, tlen, nA, dm, nullptr, nullptr, mn, tshift);
@@ -517,12 +555,16 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
             atomicOr(err, ERR_INTERNAL);
         __syncthreads();
         if (row >= 0 && ok) {
+            const int nlong = *lcnt;
             if (dense)
-                add_products<G, SLOT_DENSE, P>(op, b0, blen, av, nA, nullptr, nullptr, fm, foff, vals, mn, bshift);
+                add_products<G, SLOT_DENSE, P>(op, b0, blen, av, nA, lidx, nlong, nullptr, nullptr, fm, foff, vals, mn,
+                                               bshift);
             else if (sparse)
-                add_products<G, SLOT_TWO_LEVEL, P>(op, b0, blen, av, nA, dm, doff, fm, foff, vals, mn, bshift);
+                add_products<G, SLOT_TWO_LEVEL, P>(op, b0, blen, av, nA, lidx, nlong, dm, doff, fm, foff, vals, mn,
+                                                   bshift);
             else
-                add_products<G, SLOT_FLAT, P>(op, b0, blen, av, nA, nullptr, nullptr, fm, foff, vals, mn, bshift);
+                add_products<G, SLOT_FLAT, P>(op, b0, blen, av, nA, lidx, nlong, nullptr, nullptr, fm, foff, vals, mn,
+                                              bshift);
         }
         __syncthreads();
         if (row >= 0 && ok) {
@@ -562,98 +604,360 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
     }
 }
 
-// Rows that do not fit LDS: directory (masks + offsets) in a per-workgroup global
-// workspace, accumulation with global atomics straight into C's value array (FP32C: the
-// compensation terms go to comp[], folded into C at the end of the row).
+// Rows that do not fit LDS (wide rows): a dense accumulator over a window of WIN_COLS
+// columns in LDS, slid over the row's column span.  One 1024-lane workgroup per row.  Every
+// A entry e keeps, in a global workspace, a cursor into B_j (cur), the end of B_j (kend)
+// and the column at the cursor (nxt).  For each window, the entries with nxt inside it are
+// collected into an LDS list (ACT_CHUNK at a time), so an entry costs one coalesced load
+// in a window it has nothing for; the listed entries' sub-groups walk B_j from the cursor
+// while the columns stay in the window, adding each product into its slot (ds_add) and
+// marking the column present (ds_or).  Every product is read once and sorted B rows need
+// no search.  After a window its present columns are counted and written in column order
+// after those of the previous windows.  (Reference: the spECK dense path of the wide-row
+// bin, src/external/spECK/dense_acc.cu:3-267, whose early return leaves long rows
+// unwritten.)
+// Walk B_j from element k while the columns stay below chi, with a group of gw lanes of
+// one wave (this lane is gl-th, the group starts at wave lane glo, gmask = gw one-bits):
+// U elements per lane per step, all loads of a step issued before any is used.  B_j is
+// sorted, so the in-window elements of a step are a prefix of its U * gw elements; add()
+// takes each of them.  Returns the new cursor (the same in every lane of the group).
+template <int U, typename V, typename F>
+DEVI int walk_segment(const int *__restrict__ Bcol, const V *__restrict__ Bval, int k, int ke, int chi, int gw, int gl,
+                      int glo, u64 gmask, F &&add)
+{
+    while (k < ke) {
+        int c[U];
+        V v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k + gl + u * gw;
+            c[u] = INT_MAX;
+            v[u] = 0;
+            if (kk < ke) {
+                c[u] = Bcol[kk];
+                v[u] = Bval[kk];
+            }
+        }
+        int took = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool in = c[u] < chi;
+            if (in)
+                add(c[u], v[u]);
+            took += __popcll((__ballot(in) >> glo) & gmask);
+        }
+        k += took;
+        if (took < U * gw)
+            break;
+    }
+    return k;
+}
+
+constexpr int WIN_COLS = 8192;
+constexpr int WIN_WORDS = WIN_COLS / 64;
+constexpr int ACT_CHUNK = 2048;
+constexpr int WIN_LDS = WIN_COLS * 8 + WIN_WORDS * 8 + ACT_CHUNK * 4; // slots, presence bits, active list
+
+// List the entries of [b0, min(b0 + ACT_CHUNK, nA)) whose next column is below chi: those
+// with at least LONG_SEG elements of B_j left from the back of act (count *nlong), the
+// others from the front (count *nact).  Every thread of the workgroup calls it.
+DEVI void collect_active(const int *nxt, const int *cur, const int *kend, int b0, int nA, int chi, int *act, int *nact,
+                         int *nlong)
+{
+    const int l = threadIdx.x, lane = l & 63;
+    const int lim = b0 + ACT_CHUNK < nA ? b0 + ACT_CHUNK : nA;
+    for (int e0 = b0; e0 < lim; e0 += 1024) {
+        const int e = e0 + l;
+        const bool live = e < lim && nxt[e] < chi;
+        const bool lng = live && kend[e] - cur[e] >= LONG_SEG;
+        const u64 ms = __ballot(live && !lng), ml = __ballot(lng);
+        int s0 = 0, l0 = 0;
+        if (lane == 0) {
+            if (ms)
+                s0 = atomicAdd(nact, __popcll(ms));
+            if (ml)
+                l0 = atomicAdd(nlong, __popcll(ml));
+        }
+        s0 = __shfl(s0, 0, 64);
+        l0 = __shfl(l0, 0, 64);
+        if (live && !lng)
+            act[s0 + mbcnt(ms)] = e;
+        if (lng)
+            act[ACT_CHUNK - 1 - (l0 + mbcnt(ml))] = e;
+    }
+}
+
 template <typename P>
-__global__ __launch_bounds__(1024) void k_numeric_global(const int *__restrict__ rows, const int *__restrict__ count_p,
+__global__ __launch_bounds__(1024) void k_numeric_window(const int *__restrict__ rows, const int *__restrict__ count_p,
                                                          Operands op, const long long *__restrict__ crp,
-                                                         int *__restrict__ Ccol, typename P::V *Cval, float *comp,
-                                                         int bshift, int tshift, u64 *gws, long long gws_stride,
-                                                         long long gws_tiles, int *err)
+                                                         int *__restrict__ Ccol, typename P::V *__restrict__ Cval,
+                                                         int bshift, int *cur_ws, long long cur_stride, long long ft_stride,
+                                                         int *err)
 {
     typedef typename P::V V;
     const V *__restrict__ Aval = (const V *)op.Aval;
     const V *__restrict__ Bval = (const V *)op.Bval;
+    extern __shared__ __align__(16) unsigned char lds[];
+    double *acc = (double *)lds;
+    u64 *bits = (u64 *)(lds + WIN_COLS * 8);
+    int *act = (int *)(lds + WIN_COLS * 8 + WIN_WORDS * 8);
     __shared__ unsigned red[16];
-    __shared__ int ok_s;
-    const int l = threadIdx.x;
-    // workgroup region: gws_tiles masks, then gws_tiles 32-bit offsets
-    u64 *dm = gws + (size_t)blockIdx.x * gws_stride;
-    unsigned *doff = (unsigned *)(dm + gws_tiles);
+    __shared__ unsigned woff[WIN_WORDS];
+    __shared__ int nact_s, nlong_s;
+    const int l = threadIdx.x, lane = l & 63, w = l >> 6;
+    // per-workgroup workspace: cursors, ends of the B rows, columns at the cursors
+    int *cur = cur_ws + (size_t)blockIdx.x * (3 * cur_stride + ft_stride);
+    int *kend = cur + cur_stride;
+    int *nxt = kend + cur_stride;
+    const int SG = 1 << bshift, sl = l & (SG - 1), NSG = 1024 >> bshift;
+    const int sg0 = lane & ~(SG - 1); // first lane of this lane's sub-group within the wave
+    const u64 sgmask = SG >= 64 ? ~0ull : ((1ull << SG) - 1ull);
     const int nrows = *count_p;
     for (int idx = blockIdx.x; idx < nrows; idx += gridDim.x) {
         const int row = rows[idx];
+        const int a0 = op.Arp[row], nA = op.Arp[row + 1] - a0;
+        const long long cb = crp[row];
+        const long long n = crp[row + 1] - cb;
+        const int cbeg = op.rmin[row] << 6, cend = (op.rmax[row] + 1) << 6;
+        for (int e = l; e < nA; e += 1024) {
+            const int j = op.Acol[a0 + e];
+            const int k0 = op.Brp[j], k1 = op.Brp[j + 1];
+            cur[e] = k0;
+            kend[e] = k1;
+            nxt[e] = k0 < k1 ? op.Bcol[k0] : INT_MAX;
+        }
+        block_sync_global();
+        long long written = 0;
+        for (int c0 = cbeg; c0 < cend; c0 += WIN_COLS) {
+            const int c1 = c0 + WIN_COLS;
+            for (int q = l; q < WIN_COLS; q += 1024)
+                acc[q] = 0.0;
+            if (l < WIN_WORDS)
+                bits[l] = 0;
+            auto add_d = [&](int c, V v, double a) {
+                const int q = c - c0;
+                P::add(acc, (unsigned)q, a, v);
+                atomicOr(&bits[q >> 6], 1ull << (q & 63));
+            };
+            for (int base = 0; base < nA; base += ACT_CHUNK) {
+                if (l == 0) {
+                    nact_s = 0;
+                    nlong_s = 0;
+                }
+                __syncthreads();
+                collect_active(nxt, cur, kend, base, nA, c1, act, &nact_s, &nlong_s);
+                __syncthreads();
+                const int nact = nact_s, nlong = nlong_s;
+                for (int t = w; t < nlong; t += 16) {
+                    const int e = act[ACT_CHUNK - 1 - t];
+                    const int ke = kend[e];
+                    const double a = (double)Aval[a0 + e];
+                    const int k = walk_segment<4>(op.Bcol, Bval, cur[e], ke, c1, 64, lane, 0, ~0ull,
+                                                  [&](int c, V v) { add_d(c, v, a); });
+                    if (lane == 0) {
+                        cur[e] = k;
+                        nxt[e] = k < ke ? op.Bcol[k] : INT_MAX;
+                    }
+                }
+                for (int t = l >> bshift; t < nact; t += NSG) {
+                    const int e = act[t];
+                    const int ke = kend[e];
+                    const double a = (double)Aval[a0 + e];
+                    const int k = walk_segment<2>(op.Bcol, Bval, cur[e], ke, c1, SG, sl, sg0, sgmask,
+                                                  [&](int c, V v) { add_d(c, v, a); });
+                    if (sl == 0) {
+                        cur[e] = k;
+                        nxt[e] = k < ke ? op.Bcol[k] : INT_MAX;
+                    }
+                }
+                __syncthreads(); // act is rewritten by the next chunk
+            }
+            // column order: word offsets within the window, then one wave per word
+            const unsigned cnt = l < WIN_WORDS ? (unsigned)__popcll(bits[l]) : 0u;
+            unsigned tot;
+            const unsigned pre = group_excl_scan<1024>(cnt, red, tot);
+            if (l < WIN_WORDS)
+                woff[l] = pre;
+            __syncthreads();
+            for (int q = w; q < WIN_WORDS; q += 16) {
+                const u64 m = bits[q];
+                const long long r = written + woff[q] + __popcll(m & below(lane));
+                if (((m >> lane) & 1ull) && r < n) { // r >= n only if the counts disagree (flagged below)
+                    Ccol[cb + r] = c0 + (q << 6) + lane;
+                    Cval[cb + r] = P::result(acc, (unsigned)((q << 6) + lane));
+                }
+            }
+            written += tot;
+            __syncthreads();
+        }
+        if (l == 0 && written != n)
+            atomicOr(err, ERR_INTERNAL);
+        block_sync_global(); // workspace of the next row
+    }
+}
+
+// Wide rows whose tile directory was kept (flat, or two-level when the symbolic phase
+// held the row in LDS): the HSMU compact accumulator, windowed over the row's OUTPUT
+// positions.  A window is a run of directory entries [k0, k1) holding at most CW_VALS of
+// C's entries and CW_TILES entries, so its slots and its part of the directory fit LDS;
+// a column's slot is found in O(1) from the directory as in the LDS kernels (two-level
+// rows read their coarse words, a few KiB per row, from global memory).  Cursors into
+// the B rows and active lists as in the dense window kernel; a window spans the row's
+// columns from its first to its last non-empty tile, so sparse stretches cost nothing and
+// a row needs about max(nnz / CW_VALS, entries / CW_TILES) windows instead of span /
+// WIN_COLS.  Values go out coalesced (they are C's entries in order), columns from the
+// masks.
+constexpr int CW_VALS = 4096;
+constexpr int CW_TILES = 2048;
+constexpr int CW_LDS = CW_VALS * 8 + CW_TILES * 16 + ACT_CHUNK * 4; // slots, window directory, active list
+
+template <typename P>
+__global__ __launch_bounds__(1024) void k_numeric_wcompact(const int *__restrict__ rows, const int *__restrict__ count_p,
+                                                           Operands op, const long long *__restrict__ crp,
+                                                           int *__restrict__ Ccol, typename P::V *__restrict__ Cval,
+                                                           int bshift, int *ws, long long cur_stride, long long ft_stride,
+                                                           long long sym_row_max, int sym_all_global, int *err)
+{
+    typedef typename P::V V;
+    const V *__restrict__ Aval = (const V *)op.Aval;
+    const V *__restrict__ Bval = (const V *)op.Bval;
+    extern __shared__ __align__(16) unsigned char lds[];
+    double *vals = (double *)lds;
+    u64 *wm = (u64 *)(lds + CW_VALS * 8);                        // window masks
+    unsigned *wo = (unsigned *)(lds + CW_VALS * 8 + CW_TILES * 8); // slots relative to the window
+    unsigned *wt = wo + CW_TILES;                                 // tiles relative to rmin
+    int *act = (int *)(wt + CW_TILES);
+    __shared__ int nact_s, nlong_s, k1_s;
+    const int l = threadIdx.x, lane = l & 63, w = l >> 6;
+    int *cur = ws + (size_t)blockIdx.x * (3 * cur_stride + ft_stride);
+    int *kend = cur + cur_stride;
+    int *nxt = kend + cur_stride;
+    int *ft = nxt + cur_stride; // tile of every fine entry of a two-level row
+    const int SG = 1 << bshift, sl = l & (SG - 1), NSG = 1024 >> bshift;
+    const int sg0 = lane & ~(SG - 1);
+    const u64 sgmask = SG >= 64 ? ~0ull : ((1ull << SG) - 1ull);
+    const int nrows = *count_p;
+    for (int idx = blockIdx.x; idx < nrows; idx += gridDim.x) {
+        const int row = rows[idx];
+        const int a0 = op.Arp[row], nA = op.Arp[row + 1] - a0;
+        const long long cb = crp[row];
+        const long long n = crp[row + 1] - cb;
         const int mn = op.rmin[row];
         const int S = op.rmax[row] - mn + 1;
-        const long long cb = crp[row];
-        const int n = (int)(crp[row + 1] - cb);
-        for (int t = l; t < S; t += 1024)
-            dm[t] = 0;
-        for (int q = l; q < n; q += 1024) {
-            Cval[cb + q] = 0;
-            if (comp)
-                comp[cb + q] = 0.0f;
-        }
-        block_sync_global();
-        {
-            const int SG = 1 << tshift, sl = l & (SG - 1), sg = l >> tshift, NSG = 1024 >> tshift;
-            const int a1 = op.Arp[row + 1];
-            for (int e = op.Arp[row] + sg; e < a1; e += NSG) {
-                const int j = op.Acol[e];
-                const int t1 = op.tptr[j + 1];
-                for (int t = op.tptr[j] + sl; t < t1; t += SG)
-                    atomicOr(&dm[op.tcol[t] - mn], op.tmask[t]);
-            }
-        }
-        block_sync_global();
-        const int chunk = (S + 1023) / 1024;
-        const int ta = l * chunk;
-        const int tb = ta + chunk < S ? ta + chunk : S;
-        unsigned loc = 0;
-        for (int t = ta; t < tb; ++t)
-            loc += (unsigned)__popcll(dm[t]);
-        unsigned tot;
-        unsigned pre = group_excl_scan<1024>(loc, red, tot);
-        for (int t = ta; t < tb; ++t) {
-            doff[t] = pre;
-            pre += (unsigned)__popcll(dm[t]);
-        }
-        if (l == 0) {
-            ok_s = (int)tot == n;
-            if ((int)tot != n)
-                atomicOr(err, ERR_INTERNAL);
-        }
-        block_sync_global();
-        if (ok_s) {
-            const int SG = 1 << bshift, sl = l & (SG - 1), sg = l >> bshift, NSG = 1024 >> bshift;
-            const int a1 = op.Arp[row + 1];
-            for (int e = op.Arp[row] + sg; e < a1; e += NSG) {
-                const int j = op.Acol[e];
-                const double a = (double)Aval[e];
-                const int b1 = op.Brp[j + 1];
-                for (int k = op.Brp[j] + sl; k < b1; k += SG) {
-                    const int c = op.Bcol[k];
-                    const int t = (c >> 6) - mn;
-                    const unsigned slot = doff[t] + (unsigned)__popcll(dm[t] & below(c & 63));
-                    P::gadd(Cval, comp, cb + slot, a, Bval[k]);
+        const int tp = op.rtp[row];
+        const long long rd = op.rdir[row];
+        // the form the symbolic phase kept: two-level only if the row was in an LDS bin there
+        const bool two = use_sparse(S, tp) && !sym_all_global && sym_need(S, nA, tp) <= sym_row_max;
+        const int W = two ? (S + 63) >> 6 : 0;
+        const u64 *cmask = op.dir_mask + rd;      // coarse words (two-level)
+        const unsigned *coff = op.dir_off + rd;
+        const u64 *fm = op.dir_mask + rd + W;     // fine entries (flat: every tile of the span)
+        const unsigned *foff = op.dir_off + rd + W;
+        const int F = two ? (int)(coff[W - 1] + (unsigned)__popcll(cmask[W - 1])) : S;
+        if (two) {
+            for (int q = l; q < W; q += 1024) {
+                u64 m = cmask[q];
+                unsigned k = coff[q];
+                while (m) {
+                    ft[k++] = (q << 6) + __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
                 }
             }
-            const int w = l >> 6, lane = l & 63;
-            for (int t = w; t < S; t += 16) {
-                const u64 m = dm[t];
-                if ((m >> lane) & 1ull) {
-                    const unsigned pos = doff[t] + (unsigned)__popcll(m & below(lane));
-                    Ccol[cb + pos] = ((mn + t) << 6) + lane;
-                }
-            }
-            if (comp) {
-                block_sync_global();
-                for (int q = l; q < n; q += 1024)
-                    Cval[cb + q] = (V)(Cval[cb + q] + comp[cb + q]);
-            }
+        }
+        for (int e = l; e < nA; e += 1024) {
+            const int j = op.Acol[a0 + e];
+            const int k0 = op.Brp[j], k1 = op.Brp[j + 1];
+            cur[e] = k0;
+            kend[e] = k1;
+            nxt[e] = k0 < k1 ? op.Bcol[k0] : INT_MAX;
         }
         block_sync_global();
+        long long done = 0;
+        for (int k0 = 0; k0 < F;) {
+            // window end: at most CW_TILES entries and CW_VALS slots (one entry holds <= 64)
+            if (l == 0) {
+                const unsigned base = foff[k0];
+                int lo = k0 + 1, hi = k0 + CW_TILES < F ? k0 + CW_TILES : F;
+                while (lo < hi) { // largest k1 in [k0+1, hi] with slots(k0, k1) <= CW_VALS
+                    const int mid = (lo + hi + 1) >> 1;
+                    const long long end = mid == F ? n : (long long)foff[mid];
+                    if (end - base <= CW_VALS)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                k1_s = lo;
+            }
+            __syncthreads();
+            const int k1 = k1_s;
+            const int nk = k1 - k0;
+            const long long base = foff[k0];
+            const int cnt = (int)((k1 == F ? n : (long long)foff[k1]) - base);
+            for (int q = l; q < nk; q += 1024) {
+                wm[q] = fm[k0 + q];
+                wo[q] = (unsigned)(foff[k0 + q] - base);
+                wt[q] = two ? (unsigned)ft[k0 + q] : (unsigned)(k0 + q);
+            }
+            for (int q = l; q < cnt; q += 1024)
+                vals[q] = 0.0;
+            __syncthreads();
+            const int chi = (mn + (int)wt[nk - 1] + 1) << 6; // end of the window's last tile
+            auto add_c = [&](int c, V v, double a) {
+                const int tt = (c >> 6) - mn;
+                const int ke2 = two ? (int)(coff[tt >> 6] + (unsigned)__popcll(cmask[tt >> 6] & below(tt & 63))) : tt;
+                const int r = ke2 - k0;
+                P::add(vals, wo[r] + (unsigned)__popcll(wm[r] & below(c & 63)), a, v);
+            };
+            for (int b0 = 0; b0 < nA; b0 += ACT_CHUNK) {
+                if (l == 0) {
+                    nact_s = 0;
+                    nlong_s = 0;
+                }
+                __syncthreads();
+                collect_active(nxt, cur, kend, b0, nA, chi, act, &nact_s, &nlong_s);
+                __syncthreads();
+                const int nact = nact_s, nlong = nlong_s;
+                // long B rows: one wave each
+                for (int t = w; t < nlong; t += 16) {
+                    const int e = act[ACT_CHUNK - 1 - t];
+                    const int ke = kend[e];
+                    const double a = (double)Aval[a0 + e];
+                    const int k = walk_segment<4>(op.Bcol, Bval, cur[e], ke, chi, 64, lane, 0, ~0ull,
+                                                  [&](int c, V v) { add_c(c, v, a); });
+                    if (lane == 0) {
+                        cur[e] = k;
+                        nxt[e] = k < ke ? op.Bcol[k] : INT_MAX;
+                    }
+                }
+                // the others: sub-groups of SG lanes
+                for (int t = l >> bshift; t < nact; t += NSG) {
+                    const int e = act[t];
+                    const int ke = kend[e];
+                    const double a = (double)Aval[a0 + e];
+                    const int k = walk_segment<2>(op.Bcol, Bval, cur[e], ke, chi, SG, sl, sg0, sgmask,
+                                                  [&](int c, V v) { add_c(c, v, a); });
+                    if (sl == 0) {
+                        cur[e] = k;
+                        nxt[e] = k < ke ? op.Bcol[k] : INT_MAX;
+                    }
+                }
+                __syncthreads();
+            }
+            // C's entries of this window, in order
+            for (int q = l; q < cnt; q += 1024)
+                Cval[cb + base + q] = P::result(vals, (unsigned)q);
+            for (int q = w; q < nk; q += 16) {
+                const u64 m = wm[q];
+                if ((m >> lane) & 1ull)
+                    Ccol[cb + base + wo[q] + __popcll(m & below(lane))] = ((mn + (int)wt[q]) << 6) + lane;
+            }
+            done += cnt;
+            k0 = k1;
+            __syncthreads();
+        }
+        if (l == 0 && done != n)
+            atomicOr(err, ERR_INTERNAL);
+        block_sync_global(); // workspace of the next row
     }
 }
 
@@ -737,9 +1041,9 @@ static void numeric_g(const int *rows, const int *count_dev, int count, int max_
 
 template <typename P>
 static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
-                            const long long *crp, int *Ccol, void *Cval, float *comp, int bshift, int tshift,
-                            int two_level, u64 *gws, long long gws_stride, long long gws_tiles, int gws_blocks, int *err,
-                            int num_cus, unsigned *queue, hipStream_t s)
+                            const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int two_level,
+                            int *cur_ws, long long cur_stride, long long ft_stride, int win_blocks, long long sym_row_max,
+                            int sym_all_global, int *err, int num_cus, unsigned *queue, hipStream_t s)
 {
     switch (bin) {
     case 0: numeric_g<8, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
@@ -751,10 +1055,14 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
     case 6: numeric_g<512, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
     case 7: numeric_g<1024, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
     case BIN_GLOBAL: {
-        const int blocks = count < gws_blocks ? count : gws_blocks;
-        hipLaunchKernelGGL(k_numeric_global<P>, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, crp, Ccol,
-                           (typename P::V *)Cval, comp, bshift > 10 ? 10 : bshift, tshift > 10 ? 10 : tshift, gws,
-                           gws_stride, gws_tiles, err);
+        const int blocks = count < win_blocks ? count : win_blocks;
+        if (op.dir_mask)
+            hipLaunchKernelGGL(k_numeric_wcompact<P>, dim3(blocks), dim3(1024), CW_LDS, s, rows, count_dev, op, crp,
+                               Ccol, (typename P::V *)Cval, bshift > 6 ? 6 : bshift, cur_ws, cur_stride, ft_stride,
+                               sym_row_max, sym_all_global, err);
+        else
+            hipLaunchKernelGGL(k_numeric_window<P>, dim3(blocks), dim3(1024), WIN_LDS, s, rows, count_dev, op, crp, Ccol,
+                               (typename P::V *)Cval, bshift > 6 ? 6 : bshift, cur_ws, cur_stride, ft_stride, err);
         break;
     }
     default: return hipErrorInvalidValue;
@@ -763,17 +1071,23 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
 }
 
 hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count, int max_need,
-                          const Operands &op, const long long *crp, int *Ccol, void *Cval, float *comp, int bshift,
-                          int tshift, int two_level, u64 *gws, long long gws_stride, long long gws_tiles,
-                          int gws_blocks, int *err, int num_cus, unsigned *queue, hipStream_t s)
+                          const Operands &op, const long long *crp, int *Ccol, void *Cval, int bshift, int tshift,
+                          int two_level, const WideRows &wide, int *err, int num_cus, unsigned *queue, hipStream_t s)
 {
     if (count <= 0)
         return hipSuccess;
     if (fp32c)
-        return numeric_p<PrecF32C>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, comp, bshift, tshift,
-                                   two_level, gws, gws_stride, gws_tiles, gws_blocks, err, num_cus, queue, s);
-    return numeric_p<PrecF64>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, nullptr, bshift, tshift,
-                              two_level, gws, gws_stride, gws_tiles, gws_blocks, err, num_cus, queue, s);
+        return numeric_p<PrecF32C>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level,
+                                   wide.ws, wide.cur_stride, wide.ft_stride, wide.blocks, wide.sym_row_max,
+                                   wide.sym_all_global, err, num_cus, queue, s);
+    return numeric_p<PrecF64>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level,
+                              wide.ws, wide.cur_stride, wide.ft_stride, wide.blocks, wide.sym_row_max,
+                              wide.sym_all_global, err, num_cus, queue, s);
+}
+
+size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks)
+{
+    return (size_t)blocks * (size_t)(3 * cur_stride + ft_stride);
 }
 
 hipError_t configure_kernels(int lds_max)
@@ -792,6 +1106,8 @@ hipError_t configure_kernels(int lds_max)
     CFG((k_numeric_lds<128, P>)) CFG((k_numeric_lds<256, P>)) CFG((k_numeric_lds<512, P>))                           \
     CFG((k_numeric_lds<1024, P>))
     CFGP(PrecF64) CFGP(PrecF32C)
+    CFG(k_numeric_window<PrecF64>) CFG(k_numeric_window<PrecF32C>)
+    CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>)
 #undef CFGP
 #undef CFG
     return e;

@@ -495,6 +495,7 @@ struct hsmu_plan {
     DevStats *dst = nullptr;      // device counters of the plan's runs
     hsmu_stats stats0;            // statistics of the recording run
     bool empty = false;           // product with no entries: nothing to replay
+    bool dirs_kept = false;       // the recording run kept the row directories
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -645,6 +646,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     if (bp.lds_block_budget > lds_dyn)
         bp.lds_block_budget = lds_dyn;
     bp.per_lane = opts.tiles_per_lane > 0 ? opts.tiles_per_lane : 4;
+    const long long bp_sym_row_max = bp.lds_row_max;
     bp.force_global = opts.force_global;
     bp.pad = 0;
     HIP_TRY(launch_bin(0, M, P.Arp, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, s));
@@ -668,25 +670,6 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         tlanes = 64;
     const int bshift = ilog2(blanes), tshift = ilog2(tlanes);
 
-    // global workspace for rows that do not fit LDS
-    int max_span = 1;
-    for (int b = 0; b < NBINS; ++b)
-        max_span = h1.sym.max_span[b] > max_span ? h1.sym.max_span[b] : max_span;
-    const long long gws_tiles = max_span;
-    const long long gws_stride = (gws_tiles + (gws_tiles + 1) / 2 + 31) & ~31LL; // u64 units, 256 B aligned
-    int gws_blocks = c->num_cus * 2;
-    unsigned long long *gws = nullptr;
-    auto need_gws = [&](int count) -> int {
-        if (gws)
-            return HSMU_OK;
-        int blocks = count < gws_blocks ? count : gws_blocks;
-        if (blocks < 1)
-            blocks = 1;
-        gws_blocks = blocks;
-        gws = ar.get<unsigned long long>((size_t)gws_stride * (size_t)blocks);
-        return gws ? HSMU_OK : fail(HSMU_ERR_NOMEM, "global accumulator workspace");
-    };
-
     Operands op;
     op.Arp = P.Arp;
     op.Acol = P.Acol;
@@ -701,20 +684,47 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     op.rmin = rmin;
     op.rmax = rmax;
     op.rtp = rtp;
-    // Row directories (tile masks + slot offsets) of LDS-bin rows are kept from the
-    // symbolic to the numeric phase when they fit the budget (12 bytes per spanned tile:
-    // 15 MB on the banded workload); otherwise the numeric kernel rebuilds them.
+    // Row directories (tile masks + slot offsets) are kept from the symbolic to the
+    // numeric phase (12 bytes per directory entry: 15 MB on the banded workload, 36 GB for
+    // R-MAT scale 20) when they fit a quarter of the free device memory; otherwise the
+    // numeric LDS kernels rebuild them and wide rows take the dense window kernel.  A
+    // plan's capture pass repeats the recorded decision.
     op.rdir = rdir;
     op.dir_mask = nullptr;
     op.dir_off = nullptr;
     const unsigned long long dir_total = h1.dir_total;
-    const unsigned long long dir_budget = 1ull << 31;
-    if (!opts.rebuild_dirs && dir_total > 0 && dir_total * 12ull <= dir_budget) {
+    bool keep_dirs;
+    if (cap) {
+        keep_dirs = mode.plan->dirs_kept;
+    } else {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            (void)hipGetLastError();
+            free_b = 0;
+        }
+        const unsigned long long budget = free_b / 4 > (1ull << 31) ? free_b / 4 : (1ull << 31);
+        keep_dirs = !opts.rebuild_dirs && dir_total > 0 && dir_total * 12ull <= budget;
+        if (mode.plan)
+            mode.plan->dirs_kept = keep_dirs;
+    }
+    if (keep_dirs) {
         op.dir_mask = ar.get<unsigned long long>((size_t)dir_total);
         op.dir_off = ar.get<unsigned>((size_t)dir_total);
-        if (!op.dir_mask || !op.dir_off) {
-            op.dir_mask = nullptr;
-            op.dir_off = nullptr;
+        if (!op.dir_mask || !op.dir_off)
+            return fail(HSMU_ERR_NOMEM, "row directories (%llu entries)", dir_total);
+    }
+    // tile-mask workspace of the symbolic phase for rows that do not fit LDS when the
+    // directories are not kept: one span of masks per workgroup (widest span in the bin)
+    const int gws_tiles = h1.sym.max_span[BIN_GLOBAL] > 0 ? h1.sym.max_span[BIN_GLOBAL] : 1;
+    const long long gws_stride = ((long long)gws_tiles + 31) & ~31LL; // u64 units, 256 B aligned
+    int gws_blocks = 0;
+    unsigned long long *gws = nullptr;
+    if (h1.sym.count[BIN_GLOBAL] > 0) {
+        gws_blocks = h1.sym.count[BIN_GLOBAL] < 2 * c->num_cus ? h1.sym.count[BIN_GLOBAL] : 2 * c->num_cus;
+        if (!keep_dirs) {
+            gws = ar.get<unsigned long long>((size_t)gws_stride * (size_t)gws_blocks);
+            if (!gws)
+                return fail(HSMU_ERR_NOMEM, "symbolic tile-mask workspace");
         }
     }
 
@@ -723,11 +733,6 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         const int cnt = h1.sym.count[b];
         if (cnt <= 0)
             continue;
-        if (b == BIN_GLOBAL) {
-            rc = need_gws(cnt);
-            if (rc)
-                return rc;
-        }
         HIP_TRY(launch_symbolic(b, binrows + (size_t)b * M, &dst->sym.count[b], cnt, h1.sym.max_need[b], op, rnnz,
                                 tshift, h1.sym.nsparse[b] > 0 ? 1 : 0, gws, gws_stride, c->num_cus, gws_blocks,
                                 opts.persistent ? dst->queue[0][b] : nullptr, s));
@@ -796,24 +801,24 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         const int cnt = h2.num.count[b];
         if (cnt <= 0)
             continue;
+        // wide rows: a workspace per window workgroup (two 1024-lane workgroups fit a CU's
+        // LDS) for the cursors of its row's A entries and the tiles of a two-level directory
+        WideRows wide = {};
         if (b == BIN_GLOBAL) {
-            rc = need_gws(cnt);
-            if (rc)
-                return rc;
-        }
-        // compensation terms of FP32C rows accumulated in global memory
-        float *comp = nullptr;
-        if (fp32 && b == BIN_GLOBAL) {
-            comp = ar.get<float>((size_t)nnzc);
-            if (!comp)
-                return fail(HSMU_ERR_NOMEM, "compensation scratch (nnz %lld)", nnzc);
+            wide.blocks = cnt < 2 * c->num_cus ? cnt : 2 * c->num_cus;
+            wide.cur_stride = ((long long)h2.num.max_na[b] + 63) & ~63LL;
+            wide.ft_stride = keep_dirs ? (((long long)h2.num.max_span[b] + 63) & ~63LL) : 0;
+            wide.sym_row_max = bp_sym_row_max;
+            wide.sym_all_global = opts.force_global ? 1 : 0;
+            wide.ws = ar.get<int>(wide_ws_ints(wide.cur_stride, wide.ft_stride, wide.blocks));
+            if (!wide.ws)
+                return fail(HSMU_ERR_NOMEM, "wide-row workspace");
         }
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][0], s));
         HIP_TRY(launch_numeric(b, fp32 ? 1 : 0, binrows + (size_t)b * M, &dst->num.count[b], cnt, h2.num.max_need[b], op,
-                               crp, Ccol, Cval, comp, bshift, tshift, h2.num.nsparse[b] > 0 ? 1 : 0, gws, gws_stride,
-                               gws_tiles, gws_blocks, errp, c->num_cus, opts.persistent ? dst->queue[1][b] : nullptr,
-                               s));
+                               crp, Ccol, Cval, bshift, tshift, h2.num.nsparse[b] > 0 ? 1 : 0, wide, errp, c->num_cus,
+                               opts.persistent ? dst->queue[1][b] : nullptr, s));
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][1], s));
     }

@@ -45,6 +45,7 @@ struct BinStats {
     unsigned long long nnz_c[NBINS];// nnz of C in the bin's rows (numeric)
     unsigned long long nnz_a[NBINS];// nnz of A in the bin's rows
     int nsparse[NBINS];             // rows with a two-level tile directory
+    int max_na[NBINS];              // longest row of A in the bin
 };
 
 // Device-side counters of one call, copied to pinned host memory at the two
@@ -125,9 +126,11 @@ HSMU_HD inline long long dir_entries(long long S, long long tp)
 //   t0   i32[nA]    first tile of B_j
 //   blen i32[nA]    numeric: nnz(B_j)
 //   b0   i32[nA]    numeric: first stored element of B_j
+//   lidx i32[nA]    numeric: entries whose B_j is long (walked by the whole group)
+//   lcnt i32        numeric: their number (16 bytes reserved)
 // The size is rounded up to 16 bytes so per-row slices of a workgroup stay aligned.
 struct RowLayout {
-    long long dm, doff, ft, vals, av, tlen, t0, blen, b0, bytes;
+    long long dm, doff, ft, vals, av, tlen, t0, blen, b0, lidx, lcnt, bytes;
 };
 
 HSMU_HD inline RowLayout row_layout(int numeric, long long S, long long n, long long nA, long long tp)
@@ -161,6 +164,13 @@ HSMU_HD inline RowLayout row_layout(int numeric, long long S, long long n, long 
     L.b0 = o;
     if (numeric)
         o += 4 * nA;
+    L.lidx = o;
+    if (numeric)
+        o += 4 * nA;
+    o = (o + 15) & ~15LL;
+    L.lcnt = o;
+    if (numeric)
+        o += 16;
     L.bytes = (o + 15) & ~15LL;
     return L;
 }
@@ -235,13 +245,28 @@ hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int c
 hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *scan_ws,
                                long long *total_copy, hipStream_t s);
 
+// Wide rows (BIN_GLOBAL) in the numeric phase: blocks workgroups, each with a workspace of
+// 3 * cur_stride + ft_stride ints (cursors, ends and next columns of its row's A entries:
+// cur_stride >= the longest row of A in the bin; tiles of the fine directory entries of
+// a two-level row: ft_stride >= the widest span in the bin).  With kept directories the
+// compact windowed kernel runs, and needs to know which rows the symbolic phase held in
+// global memory (their directories are flat): sym_row_max (LDS bytes a row may use) and
+// sym_all_global (every row was global: force_global).
+struct WideRows {
+    int *ws;
+    long long cur_stride, ft_stride;
+    int blocks;
+    long long sym_row_max;
+    int sym_all_global;
+};
+size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks);
+
 // Numeric phase of one bin.  fp32c = 0: FP64 values (Cval double); fp32c = 1: compensated
-// FP32 (Cval float; comp = nnz(C) floats of scratch, needed only for BIN_GLOBAL).
+// FP32 (Cval float).
 hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count,
                           int max_need, const Operands &op, const long long *crp, int *Ccol,
-                          void *Cval, float *comp, int bshift, int tshift, int two_level,
-                          unsigned long long *gws, long long gws_stride, long long gws_tiles,
-                          int gws_blocks, int *err, int num_cus, unsigned *queue, hipStream_t s);
+                          void *Cval, int bshift, int tshift, int two_level, const WideRows &wide,
+                          int *err, int num_cus, unsigned *queue, hipStream_t s);
 
 hipError_t launch_rowptr_i64_to_i32(const long long *in, int64_t n, int *out, hipStream_t s);
 

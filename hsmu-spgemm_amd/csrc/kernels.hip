@@ -14,7 +14,7 @@
 //      Reference: compressed_tilePtr_symbolic.cuh + compressed_tileOR.cuh, which do the
 //      same in two hash passes; the dense directory needs one pass and no hashing.
 //   5. Scan of nnz(C_i) to int64 row pointers.
-//   6. Numeric (k_numeric_lds / k_numeric_global): the HSMU accumulator.  The row's
+//   6. Numeric (k_numeric_lds / k_numeric_window): the HSMU accumulator.  The row's
 //      value array is exactly nnz(C_i) long; the slot of column c is
 //        off[t] + popcount(mask[t] & ((1 << (c & 63)) - 1)),  t = (c >> 6) - rmin,
 //      i.e. the rank of c among C's sorted columns, found in O(1) from the tile
@@ -23,10 +23,10 @@
 //      The same kernel writes C's sorted columns by expanding the tile masks, which
 //      replaces the reference's separate O(n^2) rank-sort pass
 //      (src/compressed/compressed_Ccol_symbolic.cuh:4-63).
-//   Rows whose directory + values do not fit LDS keep them in a global workspace and
-//   accumulate with FP64 atomics straight into C (the reference's bin-13 dense spECK
-//   path, src/external/spECK/dense_acc.cu:246-267, whose early return leaves rows
-//   unwritten; every row is written here).
+//   Rows whose directory + values do not fit LDS: the symbolic tile OR runs on a global
+//   workspace, the numeric phase slides a dense LDS window over the row's columns (the
+//   reference's bin-13 dense spECK path, src/external/spECK/dense_acc.cu:246-267, whose
+//   early return leaves rows unwritten; every row is written here).
 
 #include <hip/hip_runtime.h>
 
@@ -499,7 +499,7 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
                                               const int *__restrict__ rnnz, BinParams bp, int *__restrict__ binrows,
                                               long long *__restrict__ rdir, DevStats *st)
 {
-    __shared__ int s_cnt[NBINS], s_need[NBINS], s_span[NBINS], s_base[NBINS], s_sparse[NBINS];
+    __shared__ int s_cnt[NBINS], s_need[NBINS], s_span[NBINS], s_base[NBINS], s_sparse[NBINS], s_na_max[NBINS];
     __shared__ unsigned long long s_work[NBINS], s_nc[NBINS], s_na[NBINS];
     __shared__ unsigned long long s_dir, s_dir_base;
     if (threadIdx.x < NBINS) {
@@ -510,6 +510,7 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
         s_nc[threadIdx.x] = 0;
         s_na[threadIdx.x] = 0;
         s_sparse[threadIdx.x] = 0;
+        s_na_max[threadIdx.x] = 0;
     }
     if (threadIdx.x == 0)
         s_dir = 0;
@@ -531,8 +532,10 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
         }
         b = choose_bin(work, need, bp);
         sp = use_sparse(S, tp) ? 1 : 0;
-        if (!numeric && b < BIN_GLOBAL)
-            E = dir_entries(S, tp);
+        // directory entries kept for the numeric phase: the row's form in an LDS bin, the
+        // flat span for rows the symbolic phase handles in global memory
+        if (!numeric)
+            E = b < BIN_GLOBAL ? dir_entries(S, tp) : S;
     }
     // Aggregate per wave first: the lanes that chose one bin (neighbouring rows usually
     // share it) add their counts and sums with one LDS atomic per bin per wave, instead
@@ -552,6 +555,7 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
         const unsigned long long nas = wave_sum<unsigned long long>(mine ? (unsigned long long)nA : 0ull);
         const int needm = wave_max<int>(mine ? (int)(need < INT_MAX ? need : INT_MAX) : 0);
         const int spanm = wave_max<int>(mine ? S : 0);
+        const int nam = wave_max<int>(mine ? nA : 0);
         const int sps = wave_sum<int>(mine ? sp : 0);
         const unsigned long long dincl = wave_incl_scan<unsigned long long>(mine ? (unsigned long long)E : 0ull);
         const unsigned long long dtot = __shfl(dincl, 63, 64);
@@ -561,6 +565,7 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
             base = atomicAdd(&s_cnt[bb], __popcll(same));
             atomicMax(&s_need[bb], needm);
             atomicMax(&s_span[bb], spanm);
+            atomicMax(&s_na_max[bb], nam);
             atomicAdd(&s_work[bb], wsum);
             atomicAdd(&s_nc[bb], ncs);
             atomicAdd(&s_na[bb], nas);
@@ -585,6 +590,7 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
         if (c) {
             atomicMax(&bs->max_need[threadIdx.x], s_need[threadIdx.x]);
             atomicMax(&bs->max_span[threadIdx.x], s_span[threadIdx.x]);
+            atomicMax(&bs->max_na[threadIdx.x], s_na_max[threadIdx.x]);
             atomicAdd(&bs->work[threadIdx.x], s_work[threadIdx.x]);
             atomicAdd(&bs->nnz_c[threadIdx.x], s_nc[threadIdx.x]);
             atomicAdd(&bs->nnz_a[threadIdx.x], s_na[threadIdx.x]);
@@ -789,7 +795,7 @@ __global__ void k_plan_check(DevStats *st, const DevStats *expect, int stage)
     bool same = true;
     for (int k = 0; k < NBINS; ++k)
         same = same && a.count[k] == b.count[k] && a.max_need[k] == b.max_need[k] && a.max_span[k] == b.max_span[k] &&
-               a.nsparse[k] == b.nsparse[k];
+               a.nsparse[k] == b.nsparse[k] && a.max_na[k] == b.max_na[k];
     if (stage == 0)
         same = same && st->dir_total == expect->dir_total && st->ntiles_b == expect->ntiles_b;
     else
