@@ -51,11 +51,11 @@ DEVI void stage_row(const Operands &op, int a0, int nA, int *t0, int *tlen, int 
 
 constexpr int UNROLL = 4;
 
-// A B row with at least LONG_SEG elements (left) is walked by a whole group of lanes --
-// the row's G lanes in the LDS kernels, a wave in the window kernels -- and shorter ones by
-// sub-groups: R-MAT's hub rows of B hold tens of thousands of entries and would otherwise
-// leave one sub-group walking while the rest of the workgroup waits.
-constexpr int LONG_SEG = 256;
+// A B row with at least LONG_SEG elements (left; LONG_TILES tiles in the tile OR) is
+// walked by a whole wave, shorter ones by sub-groups: R-MAT's hub rows of B hold tens of
+// thousands of entries and would otherwise leave one sub-group walking while the rest of
+// the workgroup waits.  The LDS kernels list such entries only in their LONG variants,
+// launched when B has a row that long.
 
 // Compact index of tile t (relative to rmin) in a two-level directory: the rank of t
 // among the row's non-empty tiles.
@@ -73,41 +73,62 @@ enum { OR_FLAT, OR_COARSE, OR_FINE };
 
 // Sub-groups of 2^tshift lanes take one entry j each and walk the tiles of B_j, UNROLL
 // tiles per lane per iteration with all loads issued before the ds_or_b64s.
-template <int G, int MODE>
-DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, u64 *dm, const unsigned *coff, u64 *fm,
-                   int mn, int tshift)
+template <int MODE>
+DEVI void or_put(u64 *dm, const unsigned *coff, u64 *fm, int tt, u64 m)
 {
-    const int l = threadIdx.x & (G - 1);
-    const int SG = 1 << tshift, sl = l & (SG - 1), NSG = G >> tshift;
-    for (int e = l >> tshift; e < nA; e += NSG) {
-        const int tb = t0[e], te = tb + tlen[e];
-        for (int t = tb + sl; t < te; t += UNROLL * SG) {
-            int c[UNROLL];
-            u64 m[UNROLL];
+    if constexpr (MODE == OR_FLAT)
+        atomicOr(&dm[tt], m);
+    else if constexpr (MODE == OR_COARSE)
+        atomicOr(&dm[tt >> 6], 1ull << (tt & 63));
+    else
+        atomicOr(&fm[fine_index(dm, coff, tt)], m);
+}
+
+// Tiles [tb, te) of one B row by a group of gw lanes (this lane the gl-th).
+template <int MODE>
+DEVI void or_walk(const Operands &op, int tb, int te, int gw, int gl, u64 *dm, const unsigned *coff, u64 *fm, int mn)
+{
+    for (int t = tb + gl; t < te; t += UNROLL * gw) {
+        int c[UNROLL];
+        u64 m[UNROLL];
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                const int tu = t + u * SG;
-                c[u] = -1;
-                m[u] = 0;
-                if (tu < te) {
-                    c[u] = op.tcol[tu];
-                    if constexpr (MODE != OR_COARSE)
-                        m[u] = op.tmask[tu];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                if (c[u] >= 0) {
-                    const int tt = c[u] - mn;
-                    if constexpr (MODE == OR_FLAT)
-                        atomicOr(&dm[tt], m[u]);
-                    else if constexpr (MODE == OR_COARSE)
-                        atomicOr(&dm[tt >> 6], 1ull << (tt & 63));
-                    else
-                        atomicOr(&fm[fine_index(dm, coff, tt)], m[u]);
-                }
+        for (int u = 0; u < UNROLL; ++u) {
+            const int tu = t + u * gw;
+            c[u] = -1;
+            m[u] = 0;
+            if (tu < te) {
+                c[u] = op.tcol[tu];
+                if constexpr (MODE != OR_COARSE)
+                    m[u] = op.tmask[tu];
             }
         }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u)
+            if (c[u] >= 0)
+                or_put<MODE>(dm, coff, fm, c[u] - mn, m[u]);
+    }
+}
+
+template <int G, int MODE, bool LIST = false>
+DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, const int *lidx, int nlong, u64 *dm,
+                   const unsigned *coff, u64 *fm, int mn, int tshift)
+{
+    const int l = threadIdx.x & (G - 1);
+    // LIST: entries with many tiles (listed in lidx), one wave each (the whole group if
+    // smaller); the sub-group walks below skip them
+    if constexpr (LIST) {
+        constexpr int LW = G < 64 ? G : 64;
+        for (int i = l / LW; i < nlong; i += G / LW) {
+            const int e = lidx[i];
+            or_walk<MODE>(op, t0[e], t0[e] + tlen[e], LW, l & (LW - 1), dm, coff, fm, mn);
+        }
+    }
+    // sub-groups of 2^tshift lanes take one entry each
+    const int SG = 1 << tshift, sl = l & (SG - 1), NSG = G >> tshift;
+    for (int e = l >> tshift; e < nA; e += NSG) {
+        if (LIST && tlen[e] >= LONG_TILES)
+            continue;
+        or_walk<MODE>(op, t0[e], t0[e] + tlen[e], SG, sl, dm, coff, fm, mn);
     }
 }
 
@@ -191,74 +212,64 @@ enum { SLOT_FLAT, SLOT_DENSE, SLOT_TWO_LEVEL };
 // take one entry each and walk B_j, UNROLL elements per lane per iteration with all
 // loads issued before the ds_add_f64s.  fm/foff: the (fine) tile masks and first slots;
 // cm/coff: the coarse level of a two-level directory.
-template <int G, int FORM, typename P>
+// B_j's elements [kb, ke) by a group of gw lanes (this lane the gl-th), each product added
+// into its slot.
+template <int FORM, typename P>
+DEVI void add_walk(const Operands &op, int kb, int ke, int gw, int gl, double a, const u64 *cm, const unsigned *coff,
+                   const u64 *fm, const unsigned *foff, double *vals, int mn)
+{
+    typedef typename P::V V;
+    const V *__restrict__ Bval = (const V *)op.Bval;
+    for (int k = kb + gl; k < ke; k += UNROLL * gw) {
+        int c[UNROLL];
+        V v[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int ku = k + u * gw;
+            c[u] = -1;
+            v[u] = 0;
+            if (ku < ke) {
+                c[u] = op.Bcol[ku];
+                v[u] = Bval[ku];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            if (c[u] >= 0) {
+                unsigned slot;
+                if constexpr (FORM == SLOT_DENSE) {
+                    slot = (unsigned)(c[u] - (mn << 6));
+                } else {
+                    const int t = (c[u] >> 6) - mn;
+                    const unsigned kk = FORM == SLOT_TWO_LEVEL ? fine_index(cm, coff, t) : (unsigned)t;
+                    slot = foff[kk] + (unsigned)__popcll(fm[kk] & below(c[u] & 63));
+                }
+                P::add(vals, slot, a, v[u]);
+            }
+        }
+    }
+}
+
+template <int G, int FORM, typename P, bool LIST>
 DEVI void add_products(const Operands &op, const int *b0, const int *blen, const double *av, int nA, const int *lidx,
                        int nlong, const u64 *cm, const unsigned *coff, const u64 *fm, const unsigned *foff, double *vals,
                        int mn, int bshift)
 {
-    typedef typename P::V V;
-    const V *__restrict__ Bval = (const V *)op.Bval;
     const int l = threadIdx.x & (G - 1);
-    const int SG = 1 << bshift, sl = l & (SG - 1), NSG = G >> bshift;
-    auto add = [&](int c, V v, double a) {
-        unsigned slot;
-        if constexpr (FORM == SLOT_DENSE) {
-            slot = (unsigned)(c - (mn << 6));
-        } else {
-            const int t = (c >> 6) - mn;
-            const unsigned k = FORM == SLOT_TWO_LEVEL ? fine_index(cm, coff, t) : (unsigned)t;
-            slot = foff[k] + (unsigned)__popcll(fm[k] & below(c & 63));
-        }
-        P::add(vals, slot, a, v);
-    };
-    // long B rows (listed in lidx): all G lanes of the row, UNROLL elements per lane per step
-    for (int i = 0; i < nlong; ++i) {
-        const int e = lidx[i];
-        const int kb = b0[e], ke = kb + blen[e];
-        const double a = av[e];
-        for (int k = kb + l; k < ke; k += UNROLL * G) {
-            int c[UNROLL];
-            V v[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                const int ku = k + u * G;
-                c[u] = -1;
-                v[u] = 0;
-                if (ku < ke) {
-                    c[u] = op.Bcol[ku];
-                    v[u] = Bval[ku];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u)
-                if (c[u] >= 0)
-                    add(c[u], v[u], a);
+    // LIST: long B rows (listed in lidx), one wave each (the whole group if smaller)
+    if constexpr (LIST) {
+        constexpr int LW = G < 64 ? G : 64;
+        for (int i = l / LW; i < nlong; i += G / LW) {
+            const int e = lidx[i];
+            add_walk<FORM, P>(op, b0[e], b0[e] + blen[e], LW, l & (LW - 1), av[e], cm, coff, fm, foff, vals, mn);
         }
     }
-    // the others: sub-groups of SG lanes take one entry each
+    // sub-groups of SG lanes take one entry each
+    const int SG = 1 << bshift, sl = l & (SG - 1), NSG = G >> bshift;
     for (int e = l >> bshift; e < nA; e += NSG) {
-        const int kb = b0[e], ke = kb + blen[e];
-        if (ke - kb >= LONG_SEG)
+        if (LIST && blen[e] >= LONG_SEG)
             continue;
-        const double a = av[e];
-        for (int k = kb + sl; k < ke; k += UNROLL * SG) {
-            int c[UNROLL];
-            V v[UNROLL];
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u) {
-                const int ku = k + u * SG;
-                c[u] = -1;
-                v[u] = 0;
-                if (ku < ke) {
-                    c[u] = op.Bcol[ku];
-                    v[u] = Bval[ku];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < UNROLL; ++u)
-                if (c[u] >= 0)
-                    add(c[u], v[u], a);
-        }
+        add_walk<FORM, P>(op, b0[e], b0[e] + blen[e], SG, sl, av[e], cm, coff, fm, foff, vals, mn);
     }
 }
 
@@ -271,7 +282,7 @@ DEVI void add_products(const Operands &op, const int *b0, const int *blen, const
 // two_level = 0 when no row of the launch is two-level (its extra steps are skipped).
 // When the directory is kept, it goes out with each entry's exclusive prefix (coarse:
 // first compact index; fine or flat: first slot) for the numeric phase.
-template <int G>
+template <int G, bool LONG>
 __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const int *__restrict__ rows,
                                                                        const int *__restrict__ count_p, int slice_bytes,
                                                                        Operands op, int *__restrict__ rnnz, int tshift,
@@ -298,19 +309,34 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
         }
         const bool sparse = use_sparse(S, tp);
         const int W = sparse ? (S + 63) >> 6 : S; // entries of the first level
-        const RowLayout L = row_layout(0, S, 0, nA, tp);
+        const RowLayout L = row_layout(0, S, 0, nA, tp, LONG);
         u64 *dm = (u64 *)(slice + L.dm);
         unsigned *doff = (unsigned *)(slice + L.doff);
         int *tlen = (int *)(slice + L.tlen);
         int *t0 = (int *)(slice + L.t0);
+        int *lidx = (int *)(slice + L.lidx);
+        int *lcnt = (int *)(slice + L.lcnt);
         for (int t = l; t < W; t += G)
             dm[t] = 0;
+        if (LONG && G >= 128 && l == 0)
+            *lcnt = 0;
         stage_row<G, false>(op, a0, nA, t0, tlen, nullptr, nullptr, nullptr);
         __syncthreads();
+        // rows of the wide bins: entries with many tiles, one wave each
+        int nlong = 0;
+        if constexpr (LONG && G >= 128) {
+            for (int k = l; k < nA; k += G)
+                if (tlen[k] >= LONG_TILES)
+                    lidx[atomicAdd(lcnt, 1)] = k;
+            __syncthreads();
+            nlong = *lcnt;
+        }
+        // the list, when there is one, takes the long entries out of the sub-group walks
+        constexpr bool LIST = LONG && G >= 128;
         if (sparse)
-            or_tiles<G, OR_COARSE>(op, t0, tlen, nA, dm, nullptr, nullptr, mn, tshift);
+            or_tiles<G, OR_COARSE, LIST>(op, t0, tlen, nA, lidx, nlong, dm, nullptr, nullptr, mn, tshift);
         else
-            or_tiles<G, OR_FLAT>(op, t0, tlen, nA, dm, nullptr, nullptr, mn, tshift);
+            or_tiles<G, OR_FLAT, LIST>(op, t0, tlen, nA, lidx, nlong, dm, nullptr, nullptr, mn, tshift);
         __syncthreads();
         int ta, tb;
         unsigned tot;
@@ -346,7 +372,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
                     fm[k] = 0;
             __syncthreads();
             if (sparse)
-                or_tiles<G, OR_FINE>(op, t0, tlen, nA, dm, doff, fm, mn, tshift);
+                or_tiles<G, OR_FINE, LIST>(op, t0, tlen, nA, lidx, nlong, dm, doff, fm, mn, tshift);
             __syncthreads();
             unsigned tot2;
             unsigned pre2 = scan_popcounts<G>(fm, sparse ? (int)tot : 0, redu, tot2, ta, tb);
@@ -421,7 +447,7 @@ __global__ __launch_bounds__(1024) void k_symbolic_global(const int *__restrict_
 // one lane per tile walking its set bits.  Directory forms: flat (slot from the tile's
 // entry), dense (one slot per column of the span) and two-level (the tile's entry found
 // through the coarse bitmap first); two_level = 0 when no row of the launch uses it.
-template <int G, typename P>
+template <int G, typename P, bool LONG>
 __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int *__restrict__ rows,
                                                                       const int *__restrict__ count_p, int slice_bytes,
                                                                       Operands op, const long long *__restrict__ crp,
@@ -453,7 +479,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
         }
         const bool sparse = use_sparse(S, tp);
         const int W = sparse ? (S + 63) >> 6 : S; // entries of the first level
-        const RowLayout L = row_layout(1, S, n, nA, tp);
+        const RowLayout L = row_layout(1, S, n, nA, tp, LONG);
         u64 *dm = (u64 *)(slice + L.dm);
         unsigned *doff = (unsigned *)(slice + L.doff);
         unsigned *ft = (unsigned *)(slice + L.ft);
@@ -473,7 +499,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
         const long long rd = (row >= 0 && op.dir_mask) ? op.rdir[row] : -1;
         for (int q = l; q < nv; q += G)
             vals[q] = 0.0;
-        if (l == 0)
+        if (LONG && l == 0)
             *lcnt = 0;
         if (rd >= 0) {
             // directory kept by the symbolic phase: load its first level, stage only B's ranges
@@ -488,15 +514,17 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
             stage_row<G, true>(op, a0, nA, t0, tlen, b0, blen, av);
         }
         __syncthreads();
-        // entries with long rows of B, walked by the whole group
-        for (int k = l; k < nA; k += G)
-            if (blen[k] >= LONG_SEG)
-                lidx[atomicAdd(lcnt, 1)] = k;
+        // entries with long rows of B, walked by whole waves
+        if constexpr (LONG) {
+            for (int k = l; k < nA; k += G)
+                if (blen[k] >= LONG_SEG)
+                    lidx[atomicAdd(lcnt, 1)] = k;
+        }
         if (rd < 0) {
             if (sparse)
-                or_tiles<G, OR_COARSE>(op, t0, tlen, nA, dm, nullptr, nullptr, mn, tshift);
+                or_tiles<G, OR_COARSE>(op, t0, tlen, nA, nullptr, 0, dm, nullptr, nullptr, mn, tshift);
             else
-                or_tiles<G, OR_FLAT>(op, t0, tlen, nA, dm, nullptr, nullptr, mn, tshift);
+                or_tiles<G, OR_FLAT>(op, t0, tlen, nA, nullptr, 0, dm, nullptr, nullptr, mn, tshift);
         }
         __syncthreads();
         // first level: flat slot offsets (rebuilt) or coarse compact indices (rebuilt), and
@@ -536,7 +564,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
             }
             __syncthreads();
             if (sparse && rd < 0)
-                or_tiles<G, OR_FINE>(op, t0, tlen, nA, dm, doff, fm, mn, tshift);
+                or_tiles<G, OR_FINE>(op, t0, tlen, nA, nullptr, 0, dm, doff, fm, mn, tshift);
             __syncthreads();
             unsigned tot2;
             unsigned pre2 = scan_popcounts<G>(fm, sparse ? F : 0, redu, tot2, ta, tb);
@@ -555,16 +583,16 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
             atomicOr(err, ERR_INTERNAL);
         __syncthreads();
         if (row >= 0 && ok) {
-            const int nlong = *lcnt;
+            const int nlong = LONG ? *lcnt : 0;
             if (dense)
-                add_products<G, SLOT_DENSE, P>(op, b0, blen, av, nA, lidx, nlong, nullptr, nullptr, fm, foff, vals, mn,
-                                               bshift);
+                add_products<G, SLOT_DENSE, P, LONG>(op, b0, blen, av, nA, lidx, nlong, nullptr, nullptr, fm, foff, vals,
+                                                     mn, bshift);
             else if (sparse)
-                add_products<G, SLOT_TWO_LEVEL, P>(op, b0, blen, av, nA, lidx, nlong, dm, doff, fm, foff, vals, mn,
-                                                   bshift);
+                add_products<G, SLOT_TWO_LEVEL, P, LONG>(op, b0, blen, av, nA, lidx, nlong, dm, doff, fm, foff, vals,
+                                                         mn, bshift);
             else
-                add_products<G, SLOT_FLAT, P>(op, b0, blen, av, nA, lidx, nlong, nullptr, nullptr, fm, foff, vals, mn,
-                                              bshift);
+                add_products<G, SLOT_FLAT, P, LONG>(op, b0, blen, av, nA, lidx, nlong, nullptr, nullptr, fm, foff, vals,
+                                                    mn, bshift);
         }
         __syncthreads();
         if (row >= 0 && ok) {
@@ -816,7 +844,8 @@ __global__ __launch_bounds__(1024) void k_numeric_wcompact(const int *__restrict
                                                            Operands op, const long long *__restrict__ crp,
                                                            int *__restrict__ Ccol, typename P::V *__restrict__ Cval,
                                                            int bshift, int *ws, long long cur_stride, long long ft_stride,
-                                                           long long sym_row_max, int sym_all_global, int *err)
+                                                           long long sym_row_max, int sym_all_global, int long_rows,
+                                                           int *err)
 {
     typedef typename P::V V;
     const V *__restrict__ Aval = (const V *)op.Aval;
@@ -847,7 +876,7 @@ __global__ __launch_bounds__(1024) void k_numeric_wcompact(const int *__restrict
         const int tp = op.rtp[row];
         const long long rd = op.rdir[row];
         // the form the symbolic phase kept: two-level only if the row was in an LDS bin there
-        const bool two = use_sparse(S, tp) && !sym_all_global && sym_need(S, nA, tp) <= sym_row_max;
+        const bool two = use_sparse(S, tp) && !sym_all_global && sym_need(S, nA, tp, long_rows) <= sym_row_max;
         const int W = two ? (S + 63) >> 6 : 0;
         const u64 *cmask = op.dir_mask + rd;      // coarse words (two-level)
         const unsigned *coff = op.dir_off + rd;
@@ -983,7 +1012,7 @@ static int64_t row_grid(Kern kernel, int bs, size_t lds, int64_t groups, int num
     return blocks < 1 ? 1 : blocks;
 }
 
-template <int G>
+template <int G, bool LONG>
 static void symbolic_g(const int *rows, const int *count_dev, int count, int max_need, const Operands &op, int *rnnz,
                        int tshift, int two_level, int num_cus, unsigned *queue, hipStream_t s)
 {
@@ -991,39 +1020,47 @@ static void symbolic_g(const int *rows, const int *count_dev, int count, int max
     constexpr int RPB = BS / G;
     const int slice = (max_need + 15) & ~15;
     const size_t lds = (size_t)slice * RPB;
-    const int64_t blocks = row_grid(k_symbolic_lds<G>, BS, lds, ((int64_t)count + RPB - 1) / RPB, num_cus, queue);
+    const int64_t blocks =
+        row_grid(k_symbolic_lds<G, LONG>, BS, lds, ((int64_t)count + RPB - 1) / RPB, num_cus, queue);
     const int lg = __builtin_ctz(G);
-    hipLaunchKernelGGL((k_symbolic_lds<G>), dim3((unsigned)blocks), dim3(BS), lds, s, rows, count_dev, slice, op, rnnz,
-                       tshift > lg ? lg : tshift, two_level, queue);
+    hipLaunchKernelGGL((k_symbolic_lds<G, LONG>), dim3((unsigned)blocks), dim3(BS), lds, s, rows, count_dev, slice, op,
+                       rnnz, tshift > lg ? lg : tshift, two_level, queue);
+}
+
+template <bool LONG>
+static void symbolic_l(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
+                       int *rnnz, int tshift, int two_level, int num_cus, unsigned *queue, hipStream_t s)
+{
+#define SYM(B, G)                                                                                                      \
+    case B: symbolic_g<G, LONG>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
+    switch (bin) {
+        SYM(0, 8) SYM(1, 16) SYM(2, 32) SYM(3, 64) SYM(4, 128) SYM(5, 256) SYM(6, 512) SYM(7, 1024)
+    }
+#undef SYM
 }
 
 hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
-                           int *rnnz, int tshift, int two_level, u64 *gws, long long gws_stride, int num_cus,
-                           int gws_blocks, unsigned *queue, hipStream_t s)
+                           int *rnnz, int tshift, int two_level, int long_rows, u64 *gws, long long gws_stride,
+                           int num_cus, int gws_blocks, unsigned *queue, hipStream_t s)
 {
     if (count <= 0)
         return hipSuccess;
-    switch (bin) {
-    case 0: symbolic_g<8>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
-    case 1: symbolic_g<16>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
-    case 2: symbolic_g<32>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
-    case 3: symbolic_g<64>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
-    case 4: symbolic_g<128>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
-    case 5: symbolic_g<256>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
-    case 6: symbolic_g<512>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
-    case 7: symbolic_g<1024>(rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s); break;
-    case BIN_GLOBAL: {
+    if (bin == BIN_GLOBAL) {
         const int blocks = count < gws_blocks ? count : gws_blocks;
         hipLaunchKernelGGL(k_symbolic_global, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, rnnz,
                            tshift > 10 ? 10 : tshift, gws, gws_stride);
-        break;
-    }
-    default: return hipErrorInvalidValue;
+    } else if (bin >= 0 && bin < NLDS) {
+        if (long_rows)
+            symbolic_l<true>(bin, rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s);
+        else
+            symbolic_l<false>(bin, rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s);
+    } else {
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <int G, typename P>
+template <int G, typename P, bool LONG>
 static void numeric_g(const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
                       const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int two_level, int *err,
                       int num_cus, unsigned *queue, hipStream_t s)
@@ -1032,57 +1069,70 @@ static void numeric_g(const int *rows, const int *count_dev, int count, int max_
     constexpr int RPB = BS / G;
     const int slice = (max_need + 15) & ~15;
     const size_t lds = (size_t)slice * RPB;
-    const int64_t blocks = row_grid(k_numeric_lds<G, P>, BS, lds, ((int64_t)count + RPB - 1) / RPB, num_cus, queue);
+    const int64_t blocks =
+        row_grid(k_numeric_lds<G, P, LONG>, BS, lds, ((int64_t)count + RPB - 1) / RPB, num_cus, queue);
     const int lg = __builtin_ctz(G);
-    hipLaunchKernelGGL((k_numeric_lds<G, P>), dim3((unsigned)blocks), dim3(BS), lds, s, rows, count_dev, slice, op, crp,
-                       Ccol, (typename P::V *)Cval, bshift > lg ? lg : bshift, tshift > lg ? lg : tshift, two_level, err,
-                       queue);
+    hipLaunchKernelGGL((k_numeric_lds<G, P, LONG>), dim3((unsigned)blocks), dim3(BS), lds, s, rows, count_dev, slice,
+                       op, crp, Ccol, (typename P::V *)Cval, bshift > lg ? lg : bshift, tshift > lg ? lg : tshift,
+                       two_level, err, queue);
+}
+
+template <typename P, bool LONG>
+static void numeric_l(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
+                      const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int two_level, int *err,
+                      int num_cus, unsigned *queue, hipStream_t s)
+{
+#define NUM(B, G)                                                                                                      \
+    case B:                                                                                                            \
+        numeric_g<G, P, LONG>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err,   \
+                              num_cus, queue, s);                                                                      \
+        break;
+    switch (bin) {
+        NUM(0, 8) NUM(1, 16) NUM(2, 32) NUM(3, 64) NUM(4, 128) NUM(5, 256) NUM(6, 512) NUM(7, 1024)
+    }
+#undef NUM
 }
 
 template <typename P>
 static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
                             const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int two_level,
-                            int *cur_ws, long long cur_stride, long long ft_stride, int win_blocks, long long sym_row_max,
-                            int sym_all_global, int *err, int num_cus, unsigned *queue, hipStream_t s)
+                            int long_rows, const WideRows &wide, int *err, int num_cus, unsigned *queue, hipStream_t s)
 {
-    switch (bin) {
-    case 0: numeric_g<8, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
-    case 1: numeric_g<16, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
-    case 2: numeric_g<32, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
-    case 3: numeric_g<64, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
-    case 4: numeric_g<128, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
-    case 5: numeric_g<256, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
-    case 6: numeric_g<512, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
-    case 7: numeric_g<1024, P>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err, num_cus, queue, s); break;
-    case BIN_GLOBAL: {
-        const int blocks = count < win_blocks ? count : win_blocks;
+    if (bin == BIN_GLOBAL) {
+        const int blocks = count < wide.blocks ? count : wide.blocks;
         if (op.dir_mask)
             hipLaunchKernelGGL(k_numeric_wcompact<P>, dim3(blocks), dim3(1024), CW_LDS, s, rows, count_dev, op, crp,
-                               Ccol, (typename P::V *)Cval, bshift > 6 ? 6 : bshift, cur_ws, cur_stride, ft_stride,
-                               sym_row_max, sym_all_global, err);
+                               Ccol, (typename P::V *)Cval, bshift > 6 ? 6 : bshift, wide.ws, wide.cur_stride,
+                               wide.ft_stride, wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
         else
             hipLaunchKernelGGL(k_numeric_window<P>, dim3(blocks), dim3(1024), WIN_LDS, s, rows, count_dev, op, crp, Ccol,
-                               (typename P::V *)Cval, bshift > 6 ? 6 : bshift, cur_ws, cur_stride, ft_stride, err);
-        break;
-    }
-    default: return hipErrorInvalidValue;
+                               (typename P::V *)Cval, bshift > 6 ? 6 : bshift, wide.ws, wide.cur_stride, wide.ft_stride,
+                               err);
+    } else if (bin >= 0 && bin < NLDS) {
+        if (long_rows)
+            numeric_l<P, true>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level,
+                               err, num_cus, queue, s);
+        else
+            numeric_l<P, false>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level,
+                                err, num_cus, queue, s);
+    } else {
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count, int max_need,
                           const Operands &op, const long long *crp, int *Ccol, void *Cval, int bshift, int tshift,
-                          int two_level, const WideRows &wide, int *err, int num_cus, unsigned *queue, hipStream_t s)
+                          int two_level, int long_rows, const WideRows &wide, int *err, int num_cus, unsigned *queue,
+                          hipStream_t s)
 {
     if (count <= 0)
         return hipSuccess;
     if (fp32c)
         return numeric_p<PrecF32C>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level,
-                                   wide.ws, wide.cur_stride, wide.ft_stride, wide.blocks, wide.sym_row_max,
-                                   wide.sym_all_global, err, num_cus, queue, s);
+                                   long_rows, wide, err, num_cus, queue, s);
     return numeric_p<PrecF64>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level,
-                              wide.ws, wide.cur_stride, wide.ft_stride, wide.blocks, wide.sym_row_max,
-                              wide.sym_all_global, err, num_cus, queue, s);
+                              long_rows, wide, err, num_cus, queue, s);
 }
 
 size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks)
@@ -1099,16 +1149,20 @@ hipError_t configure_kernels(int lds_max)
         if (r != hipSuccess)                                                                                           \
             e = r;                                                                                                     \
     }
-    CFG(k_symbolic_lds<8>) CFG(k_symbolic_lds<16>) CFG(k_symbolic_lds<32>) CFG(k_symbolic_lds<64>)
-    CFG(k_symbolic_lds<128>) CFG(k_symbolic_lds<256>) CFG(k_symbolic_lds<512>) CFG(k_symbolic_lds<1024>)
-#define CFGP(P)                                                                                                        \
-    CFG((k_numeric_lds<8, P>)) CFG((k_numeric_lds<16, P>)) CFG((k_numeric_lds<32, P>)) CFG((k_numeric_lds<64, P>))     \
-    CFG((k_numeric_lds<128, P>)) CFG((k_numeric_lds<256, P>)) CFG((k_numeric_lds<512, P>))                           \
-    CFG((k_numeric_lds<1024, P>))
-    CFGP(PrecF64) CFGP(PrecF32C)
+#define CFGL(L)                                                                                                        \
+    CFG((k_symbolic_lds<8, L>)) CFG((k_symbolic_lds<16, L>)) CFG((k_symbolic_lds<32, L>))                             \
+    CFG((k_symbolic_lds<64, L>)) CFG((k_symbolic_lds<128, L>)) CFG((k_symbolic_lds<256, L>))                          \
+    CFG((k_symbolic_lds<512, L>)) CFG((k_symbolic_lds<1024, L>))
+#define CFGP(P, L)                                                                                                     \
+    CFG((k_numeric_lds<8, P, L>)) CFG((k_numeric_lds<16, P, L>)) CFG((k_numeric_lds<32, P, L>))                       \
+    CFG((k_numeric_lds<64, P, L>)) CFG((k_numeric_lds<128, P, L>)) CFG((k_numeric_lds<256, P, L>))                    \
+    CFG((k_numeric_lds<512, P, L>)) CFG((k_numeric_lds<1024, P, L>))
+    CFGL(false) CFGL(true)
+    CFGP(PrecF64, false) CFGP(PrecF64, true) CFGP(PrecF32C, false) CFGP(PrecF32C, true)
+#undef CFGP
+#undef CFGL
     CFG(k_numeric_window<PrecF64>) CFG(k_numeric_window<PrecF32C>)
     CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>)
-#undef CFGP
 #undef CFG
     return e;
 }

@@ -161,6 +161,7 @@ struct hsmu_ctx {
     hipEvent_t evb[NBINS][2] = {};
     int num_cus = 0;
     int lds_max = 0;
+    size_t total_mem = 0; // device memory (bytes)
 };
 
 extern "C" {
@@ -208,6 +209,7 @@ int hsmu_ctx_create(int device, hsmu_ctx **out)
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
     c->lds_max = (int)prop.sharedMemPerBlock;
+    c->total_mem = prop.totalGlobalMem;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void **)&c->host_stats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) {
         delete c;
@@ -669,6 +671,8 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     if (tlanes > 64)
         tlanes = 64;
     const int bshift = ilog2(blanes), tshift = ilog2(tlanes);
+    // B has rows long enough for the whole-wave walks (the binning sized LDS for them)
+    const int long_rows = h1.max_brow >= LONG_TILES ? 1 : 0;
 
     Operands op;
     op.Arp = P.Arp;
@@ -685,10 +689,13 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     op.rmax = rmax;
     op.rtp = rtp;
     // Row directories (tile masks + slot offsets) are kept from the symbolic to the
-    // numeric phase (12 bytes per directory entry: 15 MB on the banded workload, 36 GB for
-    // R-MAT scale 20) when they fit a quarter of the free device memory; otherwise the
-    // numeric LDS kernels rebuild them and wide rows take the dense window kernel.  A
-    // plan's capture pass repeats the recorded decision.
+    // numeric phase (12 bytes per directory entry) when that is cheaper than rebuilding
+    // them -- the directories hold at most half as many entries as the tile-products that
+    // rebuild them (banded: 1.3M entries for 10M tile-products; the batched Laplacians:
+    // about as many, and rebuilding is 0.6 ms faster there) -- or when wide rows need them
+    // for the compact windows, and they fit an eighth of the device memory (36 GB on an
+    // MI355X).  Otherwise the numeric LDS kernels rebuild them and wide rows take the dense
+    // window kernel.  A plan's capture pass repeats the recorded decision.
     op.rdir = rdir;
     op.dir_mask = nullptr;
     op.dir_off = nullptr;
@@ -697,13 +704,13 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     if (cap) {
         keep_dirs = mode.plan->dirs_kept;
     } else {
-        size_t free_b = 0, total_b = 0;
-        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
-            (void)hipGetLastError();
-            free_b = 0;
-        }
-        const unsigned long long budget = free_b / 4 > (1ull << 31) ? free_b / 4 : (1ull << 31);
-        keep_dirs = !opts.rebuild_dirs && dir_total > 0 && dir_total * 12ull <= budget;
+        unsigned long long tile_products = 0;
+        for (int b = 0; b < NBINS; ++b)
+            tile_products += h1.sym.work[b];
+        const bool worth = 2 * dir_total <= tile_products || h1.sym.count[BIN_GLOBAL] > 0 ||
+                           h1.num_wide_hint > 0;
+        const unsigned long long budget = c->total_mem / 8 > (1ull << 31) ? c->total_mem / 8 : (1ull << 31);
+        keep_dirs = !opts.rebuild_dirs && worth && dir_total > 0 && dir_total * 12ull <= budget;
         if (mode.plan)
             mode.plan->dirs_kept = keep_dirs;
     }
@@ -734,7 +741,8 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         if (cnt <= 0)
             continue;
         HIP_TRY(launch_symbolic(b, binrows + (size_t)b * M, &dst->sym.count[b], cnt, h1.sym.max_need[b], op, rnnz,
-                                tshift, h1.sym.nsparse[b] > 0 ? 1 : 0, gws, gws_stride, c->num_cus, gws_blocks,
+                                tshift, h1.sym.nsparse[b] > 0 ? 1 : 0, long_rows, gws, gws_stride, c->num_cus,
+                                gws_blocks,
                                 opts.persistent ? dst->queue[0][b] : nullptr, s));
     }
     // Arrays of C handed to the caller; dropped again (one-shot calls) on any failure.
@@ -810,6 +818,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             wide.ft_stride = keep_dirs ? (((long long)h2.num.max_span[b] + 63) & ~63LL) : 0;
             wide.sym_row_max = bp_sym_row_max;
             wide.sym_all_global = opts.force_global ? 1 : 0;
+            wide.long_rows = long_rows;
             wide.ws = ar.get<int>(wide_ws_ints(wide.cur_stride, wide.ft_stride, wide.blocks));
             if (!wide.ws)
                 return fail(HSMU_ERR_NOMEM, "wide-row workspace");
@@ -817,7 +826,8 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][0], s));
         HIP_TRY(launch_numeric(b, fp32 ? 1 : 0, binrows + (size_t)b * M, &dst->num.count[b], cnt, h2.num.max_need[b], op,
-                               crp, Ccol, Cval, bshift, tshift, h2.num.nsparse[b] > 0 ? 1 : 0, wide, errp, c->num_cus,
+                               crp, Ccol, Cval, bshift, tshift, h2.num.nsparse[b] > 0 ? 1 : 0, long_rows, wide, errp,
+                               c->num_cus,
                                opts.persistent ? dst->queue[1][b] : nullptr, s));
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][1], s));

@@ -56,7 +56,8 @@ struct DevStats {
     int err;
     int ntiles_b;
     int nlong; // rows of A set aside for the one-workgroup-per-row analysis
-    int pad;
+    int max_brow; // longest row of B (counted from LONG_TILES on)
+    unsigned long long num_wide_hint; // rows whose numeric need may exceed LDS (symbolic binning)
     BinStats sym;
     BinStats num;
     // work-queue counters of the persistent LDS kernels: [phase][bin][XCD]
@@ -126,14 +127,20 @@ HSMU_HD inline long long dir_entries(long long S, long long tp)
 //   t0   i32[nA]    first tile of B_j
 //   blen i32[nA]    numeric: nnz(B_j)
 //   b0   i32[nA]    numeric: first stored element of B_j
-//   lidx i32[nA]    numeric: entries whose B_j is long (walked by the whole group)
-//   lcnt i32        numeric: their number (16 bytes reserved)
+//   lidx i32[nA]    with_long: entries whose B_j (numeric) or tile row (symbolic) is
+//                   long, walked by whole waves
+//   lcnt i32        with_long: their number (16 bytes reserved)
 // The size is rounded up to 16 bytes so per-row slices of a workgroup stay aligned.
 struct RowLayout {
     long long dm, doff, ft, vals, av, tlen, t0, blen, b0, lidx, lcnt, bytes;
 };
 
-HSMU_HD inline RowLayout row_layout(int numeric, long long S, long long n, long long nA, long long tp)
+// Rows of B at least this long (LONG_TILES for the tile OR) are walked by a whole wave:
+// with_long = the longest row of B reaches LONG_TILES, so the kernels list such entries.
+constexpr int LONG_SEG = 256;
+constexpr int LONG_TILES = 128;
+
+HSMU_HD inline RowLayout row_layout(int numeric, long long S, long long n, long long nA, long long tp, bool with_long)
 {
     RowLayout L;
     const bool sparse = use_sparse(S, tp);
@@ -165,20 +172,23 @@ HSMU_HD inline RowLayout row_layout(int numeric, long long S, long long n, long 
     if (numeric)
         o += 4 * nA;
     L.lidx = o;
-    if (numeric)
+    if (with_long)
         o += 4 * nA;
     o = (o + 15) & ~15LL;
     L.lcnt = o;
-    if (numeric)
+    if (with_long)
         o += 16;
     L.bytes = (o + 15) & ~15LL;
     return L;
 }
 
-HSMU_HD inline long long sym_need(long long S, long long nA, long long tp) { return row_layout(0, S, 0, nA, tp).bytes; }
-HSMU_HD inline long long num_need(long long S, long long n, long long nA, long long tp)
+HSMU_HD inline long long sym_need(long long S, long long nA, long long tp, bool with_long)
 {
-    return row_layout(1, S, n, nA, tp).bytes;
+    return row_layout(0, S, 0, nA, tp, with_long).bytes;
+}
+HSMU_HD inline long long num_need(long long S, long long n, long long nA, long long tp, bool with_long)
+{
+    return row_layout(1, S, n, nA, tp, with_long).bytes;
 }
 
 // LDS kept back from the dynamic allocation for the kernels' static __shared__ arrays
@@ -238,7 +248,7 @@ struct Operands {
 // counters and the LDS kernels run persistent (grid from occupancy x num_cus).
 // two_level: some row of the bin has a two-level directory (BinStats::nsparse > 0).
 hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int count,
-                           int max_need, const Operands &op, int *rnnz, int tshift, int two_level,
+                           int max_need, const Operands &op, int *rnnz, int tshift, int two_level, int long_rows,
                            unsigned long long *gws, long long gws_stride, int num_cus,
                            int gws_blocks, unsigned *queue, hipStream_t s);
 
@@ -258,6 +268,7 @@ struct WideRows {
     int blocks;
     long long sym_row_max;
     int sym_all_global;
+    int long_rows; // the with_long the symbolic binning used
 };
 size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks);
 
@@ -265,7 +276,8 @@ size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks);
 // FP32 (Cval float).
 hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count,
                           int max_need, const Operands &op, const long long *crp, int *Ccol,
-                          void *Cval, int bshift, int tshift, int two_level, const WideRows &wide,
+                          void *Cval, int bshift, int tshift, int two_level, int long_rows,
+                          const WideRows &wide,
                           int *err, int num_cus, unsigned *queue, hipStream_t s);
 
 hipError_t launch_rowptr_i64_to_i32(const long long *in, int64_t n, int *out, hipStream_t s);

@@ -212,15 +212,24 @@ __global__ __launch_bounds__(256) void k_bmark(int64_t K, int64_t nnz, const int
                                                int *__restrict__ flag, DevStats *st)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j >= K)
-        return;
-    const int s = rp[j], e = rp[j + 1];
-    if (s < 0 || e < s || (int64_t)e > nnz) {
-        atomicOr(&st->err, ERR_ROWPTR);
-        return;
+    int len = 0;
+    if (j < K) {
+        const int s = rp[j], e = rp[j + 1];
+        if (s < 0 || e < s || (int64_t)e > nnz) {
+            atomicOr(&st->err, ERR_ROWPTR);
+        } else {
+            len = e - s;
+            if (e > s)
+                flag[s] = 2;
+        }
     }
-    if (e > s)
-        flag[s] = 2;
+    // longest row, counted only from LONG_TILES on (all the kernels need to know): no
+    // atomic at all for inputs without long rows, one per wave that has one otherwise
+    if (len >= LONG_TILES) {
+        const int m = wave_max<int>(len);
+        if (len == m)
+            atomicMax(&st->max_brow, len);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_bhead(int64_t nnz, int64_t N, const int *__restrict__ col, int *__restrict__ flag,
@@ -516,7 +525,9 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
         s_dir = 0;
     __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    const bool with_long = st->max_brow >= LONG_TILES;
     int b = -1, S = 0, nA = 0, nc = 0, sp = 0;
+    bool wide = false;
     long long work = 0, need = 0, E = 0;
     if (i < M && rprod[i] > 0) {
         S = rmax[i] - rmin[i] + 1;
@@ -524,14 +535,19 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
         const int tp = rtp[i];
         if (numeric) {
             nc = rnnz[i];
-            need = num_need(S, nc, nA, tp);
+            need = num_need(S, nc, nA, tp, with_long);
             work = rprod[i];
         } else {
-            need = sym_need(S, nA, tp);
+            need = sym_need(S, nA, tp, with_long);
             work = tp;
         }
         b = choose_bin(work, need, bp);
         sp = use_sparse(S, tp) ? 1 : 0;
+        if (!numeric) {
+            // the numeric need with nnz(C_i) <= min(products, 64 * span)
+            const long long nub = rprod[i] < 64LL * S ? rprod[i] : 64LL * S;
+            wide = num_need(S, nub, nA, tp, with_long) > bp.lds_row_max;
+        }
         // directory entries kept for the numeric phase: the row's form in an LDS bin, the
         // flat span for rows the symbolic phase handles in global memory
         if (!numeric)
@@ -541,6 +557,9 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
     // share it) add their counts and sums with one LDS atomic per bin per wave, instead
     // of one per lane on the same LDS word.
     const int lane = threadIdx.x & 63;
+    const u64 wides = __ballot(wide);
+    if (lane == 0 && wides)
+        atomicAdd(&st->num_wide_hint, (unsigned long long)__popcll(wides));
     int pos = 0;
     long long dloc = -1;
     u64 pending = __ballot(b >= 0);
@@ -797,7 +816,8 @@ __global__ void k_plan_check(DevStats *st, const DevStats *expect, int stage)
         same = same && a.count[k] == b.count[k] && a.max_need[k] == b.max_need[k] && a.max_span[k] == b.max_span[k] &&
                a.nsparse[k] == b.nsparse[k] && a.max_na[k] == b.max_na[k];
     if (stage == 0)
-        same = same && st->dir_total == expect->dir_total && st->ntiles_b == expect->ntiles_b;
+        same = same && st->dir_total == expect->dir_total && st->ntiles_b == expect->ntiles_b &&
+               st->max_brow == expect->max_brow;
     else
         same = same && st->nnz_c == expect->nnz_c;
     if (!same)

@@ -145,6 +145,37 @@ def test_bins_and_fallbacks(ctx, opts):
     check(ctx, A2, B, "random under forced bins", **opts)
 
 
+def long_rows_pair(seed):
+    """B (600 x 20000) whose rows 0..4 hold 3000 entries each (the rest 0..20), and A
+    (800 x 600) whose rows hit those long rows often: exercises the whole-wave walks of
+    long rows of B in every kernel."""
+    rng = np.random.default_rng(seed)
+    K, N = 600, 20000
+    rows_b = []
+    for j in range(K):
+        n = 3000 if j < 5 else int(rng.integers(0, 21))
+        rows_b.append(np.sort(rng.choice(N, size=n, replace=False)).astype(np.int32))
+    rp_b = np.zeros(K + 1, np.int32)
+    rp_b[1:] = np.cumsum([len(r) for r in rows_b])
+    B = hsmu.HostMatrix(K, N, rp_b, np.concatenate(rows_b), rng.uniform(-1, 1, int(rp_b[-1])))
+    rows_a = []
+    for i in range(800):
+        cols = set(int(c) for c in rng.choice(K, size=int(rng.integers(0, 40)), replace=False))
+        cols |= set(int(c) for c in range(5) if rng.random() < 0.5)
+        rows_a.append(np.array(sorted(cols), np.int32))
+    rp_a = np.zeros(801, np.int32)
+    rp_a[1:] = np.cumsum([len(r) for r in rows_a])
+    A = hsmu.HostMatrix(800, K, rp_a, np.concatenate(rows_a), rng.uniform(-1, 1, int(rp_a[-1])))
+    return A, B
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(force_global=1), dict(force_global=1, rebuild_dirs=1),
+                                  dict(lds_row_max=4096), dict(products_per_lane=512)])
+def test_long_rows_of_B(ctx, opts):
+    A, B = long_rows_pair(11)
+    check(ctx, A, B, f"long rows of B {opts}", **opts)
+
+
 def test_rowptr64(ctx):
     A = hsmu.gen_banded(5000, 20, 60, 300, seed=2)
     Cm, _, _ = check(ctx, A, A, "int64 row pointers", rowptr64=1)
