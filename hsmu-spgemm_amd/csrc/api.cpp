@@ -1204,6 +1204,69 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     return run_batch(c, 1, A, B, C, opts, ar, RunMode(), stats, &empty);
 }
 
+extern "C" int hsmu_spgemm_rows(hsmu_ctx *c, const hsmu_csr *A, int64_t r0, int64_t r1, const hsmu_csr *B, hsmu_csr *C,
+                                const hsmu_opts *opts_in, hsmu_stats *stats)
+{
+    if (!c || !C || !A)
+        return fail(HSMU_ERR_INVALID, "hsmu_spgemm_rows: NULL argument");
+    const hsmu_opts opts = resolve_opts(opts_in);
+    int rc = check_opts(opts);
+    if (!rc)
+        rc = check_pair(A, B, opts.precision == HSMU_FP32_COMP);
+    if (rc)
+        return rc;
+    if (r0 < 0 || r1 < r0 || r1 > A->M)
+        return fail(HSMU_ERR_INVALID, "hsmu_spgemm_rows: rows [%lld, %lld) of %lld", (long long)r0, (long long)r1,
+                    (long long)A->M);
+    HIP_TRY(hipSetDevice(c->device));
+    memset(C, 0, sizeof(*C));
+    if (stats)
+        memset(stats, 0, sizeof(*stats));
+    // the panel as a CSR matrix of its own: row pointers rebased to 0, columns and values
+    // from the panel's first entry on
+    const int *rp = (const int *)A->rowptr;
+    int ends[2];
+    HIP_TRY(hipMemcpyAsync(&ends[0], rp + r0, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&ends[1], rp + r1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (ends[0] < 0 || ends[1] < ends[0] || ends[1] > A->nnz)
+        return fail(HSMU_ERR_INDEX, "hsmu_spgemm_rows: row pointers of the panel outside [0, nnz]");
+    Arena ar(c, nullptr, false);
+    int *prp = ar.get<int>((size_t)(r1 - r0) + 1);
+    if (!prp)
+        return fail(HSMU_ERR_NOMEM, "panel row pointers");
+    HIP_TRY(launch_rowptr_rebase(rp + r0, r1 - r0 + 1, prp, c->stream));
+    const size_t vsz = opts.precision == HSMU_FP32_COMP ? 4 : 8;
+    hsmu_csr P = *A;
+    P.M = r1 - r0;
+    P.nnz = ends[1] - ends[0];
+    P.rowptr = prp;
+    P.col = A->col + ends[0];
+    P.val = (char *)A->val + (size_t)ends[0] * vsz;
+    bool empty;
+    return run_batch(c, 1, &P, B, C, opts, ar, RunMode(), stats, &empty);
+}
+
+extern "C" int hsmu_csr_row_sums(hsmu_ctx *c, const hsmu_csr *C, double *sums, double *abs_sums)
+{
+    if (!c || !C || !sums || !abs_sums)
+        return fail(HSMU_ERR_INVALID, "hsmu_csr_row_sums: NULL argument");
+    if (C->M < 0 || !C->rowptr || (C->nnz > 0 && !C->val))
+        return fail(HSMU_ERR_INVALID, "hsmu_csr_row_sums: incomplete matrix");
+    HIP_TRY(hipSetDevice(c->device));
+    Arena ar(c, nullptr, false);
+    double *d = ar.get<double>(2 * (size_t)(C->M > 0 ? C->M : 1));
+    if (!d)
+        return fail(HSMU_ERR_NOMEM, "row sums");
+    if (C->M > 0) {
+        HIP_TRY(launch_row_sums(C->M, C->rowptr, C->rowptr64, C->val, C->val32, d, d + C->M, c->stream));
+        HIP_TRY(hipMemcpyAsync(sums, d, (size_t)C->M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(abs_sums, d + C->M, (size_t)C->M * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HSMU_OK;
+}
+
 extern "C" int hsmu_spgemm_batched(hsmu_ctx *c, int64_t count, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C,
                                    const hsmu_opts *opts_in, hsmu_stats *stats)
 {

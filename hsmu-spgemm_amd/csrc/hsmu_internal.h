@@ -282,6 +282,12 @@ hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_
 
 hipError_t launch_rowptr_i64_to_i32(const long long *in, int64_t n, int *out, hipStream_t s);
 
+// out[i] = in[i] - in[0] for i < n (row pointers of a row panel, rebased to 0)
+hipError_t launch_rowptr_rebase(const int *in, int64_t n, int *out, hipStream_t s);
+// sum[i] and asum[i] = sum of the values / of their magnitudes in row i of a CSR matrix
+hipError_t launch_row_sums(int64_t M, const void *rp, int rp64, const void *val, int val32, double *sum,
+                           double *asum, hipStream_t s);
+
 // ---- batched form (kernels.hip) ----
 // Stacked row pointers of A (which = 0, nrows = total rows of A) or B (which = 1): entry g
 // of instance i = e0_i + rp_i[g - r0_i], the last entry = total entries.  Raises

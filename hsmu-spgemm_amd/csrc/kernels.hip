@@ -658,6 +658,65 @@ hipError_t launch_rowptr_i64_to_i32(const long long *in, int64_t n, int *out, hi
     return hipGetLastError();
 }
 
+// ---- row panels and row sums ------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_rowptr_rebase(const int *__restrict__ in, int64_t n, int *__restrict__ out)
+{
+    const int base = in[0];
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+        out[i] = in[i] - base;
+}
+
+hipError_t launch_rowptr_rebase(const int *in, int64_t n, int *out, hipStream_t s)
+{
+    int64_t blocks = (n + 255) / 256;
+    blocks = blocks > 8192 ? 8192 : (blocks < 1 ? 1 : blocks);
+    hipLaunchKernelGGL(k_rowptr_rebase, dim3((unsigned)blocks), dim3(256), 0, s, in, n, out);
+    return hipGetLastError();
+}
+
+// One wave per row: sum and sum of magnitudes of the row's values.
+template <typename RP, typename V>
+__global__ __launch_bounds__(256) void k_row_sums(int64_t M, const RP *__restrict__ rp, const V *__restrict__ val,
+                                                  double *__restrict__ sum, double *__restrict__ asum)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * 4;
+    for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < M; i += stride) {
+        double s = 0.0, a = 0.0;
+        for (long long k = (long long)rp[i] + lane; k < (long long)rp[i + 1]; k += 64) {
+            const double v = (double)val[k];
+            s += v;
+            a += v < 0 ? -v : v;
+        }
+        s = wave_sum<double>(s);
+        a = wave_sum<double>(a);
+        if (lane == 0) {
+            sum[i] = s;
+            asum[i] = a;
+        }
+    }
+}
+
+hipError_t launch_row_sums(int64_t M, const void *rp, int rp64, const void *val, int val32, double *sum, double *asum,
+                           hipStream_t s)
+{
+    int64_t blocks = (M + 3) / 4;
+    blocks = blocks > 16384 ? 16384 : (blocks < 1 ? 1 : blocks);
+    const dim3 g((unsigned)blocks), b(256);
+    if (rp64 && val32)
+        hipLaunchKernelGGL((k_row_sums<long long, float>), g, b, 0, s, M, (const long long *)rp, (const float *)val, sum, asum);
+    else if (rp64)
+        hipLaunchKernelGGL((k_row_sums<long long, double>), g, b, 0, s, M, (const long long *)rp, (const double *)val, sum,
+                           asum);
+    else if (val32)
+        hipLaunchKernelGGL((k_row_sums<int, float>), g, b, 0, s, M, (const int *)rp, (const float *)val, sum, asum);
+    else
+        hipLaunchKernelGGL((k_row_sums<int, double>), g, b, 0, s, M, (const int *)rp, (const double *)val, sum, asum);
+    return hipGetLastError();
+}
+
 // ---- batched form: stacking the instances and splitting C ----------------------------------
 
 // Largest i in [0, count) with off[i] + i * step <= x (off nondecreasing, off[0] = 0 <= x):

@@ -104,6 +104,9 @@ _sig = {
     "hsmu_spgemm": (C.c_int, [_P, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Opts),
                               C.POINTER(Stats)]),
     "hsmu_csr_free": (C.c_int, [_P, C.POINTER(Csr)]),
+    "hsmu_spgemm_rows": (C.c_int, [_P, C.POINTER(Csr), C.c_int64, C.c_int64, C.POINTER(Csr), C.POINTER(Csr),
+                                   C.POINTER(Opts), C.POINTER(Stats)]),
+    "hsmu_csr_row_sums": (C.c_int, [_P, C.POINTER(Csr), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "hsmu_spgemm_batched": (C.c_int, [_P, C.c_int64, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr),
                                       C.POINTER(Opts), C.POINTER(Stats)]),
     "hsmu_batch_free": (C.c_int, [_P, C.c_int64, C.POINTER(Csr)]),
@@ -333,6 +336,23 @@ class Context:
         _check(lib.hsmu_spgemm(self.handle, C.byref(A.csr), C.byref(B.csr), C.byref(c), C.byref(o), C.byref(st)),
                "hsmu_spgemm")
         return c, st.to_dict()
+
+    def spgemm_rows_device(self, A: DeviceMatrix, r0: int, r1: int, B: DeviceMatrix, opts: Opts | None = None):
+        """C = A[r0:r1, :] * B on the device (a row panel); returns (Csr, stats dict)."""
+        c = Csr()
+        st = Stats()
+        o = opts if opts is not None else default_opts()
+        _check(lib.hsmu_spgemm_rows(self.handle, C.byref(A.csr), r0, r1, C.byref(B.csr), C.byref(c), C.byref(o),
+                                    C.byref(st)), "hsmu_spgemm_rows")
+        return c, st.to_dict()
+
+    def row_sums(self, c: Csr) -> tuple[np.ndarray, np.ndarray]:
+        """(sum, sum of magnitudes) of every row of a device result."""
+        s = np.zeros(max(int(c.M), 1), np.float64)
+        a = np.zeros(max(int(c.M), 1), np.float64)
+        _check(lib.hsmu_csr_row_sums(self.handle, C.byref(c), s.ctypes.data_as(C.POINTER(C.c_double)),
+                                     a.ctypes.data_as(C.POINTER(C.c_double))), "hsmu_csr_row_sums")
+        return s[:int(c.M)], a[:int(c.M)]
 
     def free_result(self, c: Csr) -> None:
         _check(lib.hsmu_csr_free(self.handle, C.byref(c)), "hsmu_csr_free")

@@ -184,6 +184,17 @@ int hsmu_spgemm(hsmu_ctx *ctx, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C
 /* Release C's device arrays (pool or hipFree, as allocated) and zero the struct. */
 int hsmu_csr_free(hsmu_ctx *ctx, hsmu_csr *C);
 
+/* C = A[r0:r1, :] * B: the product for a panel of A's rows (C has r1 - r0 rows), for
+ * products too large for device memory, streamed a panel at a time (no counterpart in
+ * the reference, which holds the whole C on one GPU: src/NHC_spgemm.cuh:7-43). */
+int hsmu_spgemm_rows(hsmu_ctx *ctx, const hsmu_csr *A, int64_t r0, int64_t r1, const hsmu_csr *B,
+                     hsmu_csr *C, const hsmu_opts *opts, hsmu_stats *stats);
+
+/* Host arrays sums[i], abs_sums[i] (C.M each) = sum of the values, of their magnitudes,
+ * in row i of a device CSR matrix: a check of C that needs no download of C
+ * (row sums of A*B equal A times the row sums of B). */
+int hsmu_csr_row_sums(hsmu_ctx *ctx, const hsmu_csr *C, double *sums, double *abs_sums);
+
 /* B -> compressed 64-column tiles (the conversion stage on its own). */
 int hsmu_btiles(hsmu_ctx *ctx, const hsmu_csr *B, hsmu_tiles *T);
 int hsmu_tiles_free(hsmu_ctx *ctx, hsmu_tiles *T);

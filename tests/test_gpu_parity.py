@@ -176,6 +176,38 @@ def test_long_rows_of_B(ctx, opts):
     check(ctx, A, B, f"long rows of B {opts}", **opts)
 
 
+@pytest.mark.parametrize("case", ["banded", "zipf_aat"])
+def test_row_panels_and_row_sums(ctx, case):
+    """hsmu_spgemm_rows over consecutive panels reproduces the oracle's rows; the device
+    row sums equal A * rowsum(B) (the closed form the streamed workloads check with)."""
+    if case == "banded":
+        A = hsmu.gen_banded(5000, 20, 60, 300, seed=6)
+        B = A
+    else:
+        A = hsmu.gen_zipf(6000, 3000, 8, 1.2, seed=4)
+        B = hsmu.transpose(A)
+    dA, dB = ctx.matrix(A), ctx.matrix(B)
+    rb = np.bincount(np.repeat(np.arange(B.M), np.diff(B.rowptr)), weights=B.val, minlength=B.M)
+    try:
+        for r0, r1 in [(0, 1000), (1000, 2500), (2500, 2500), (2500, A.M)]:
+            note(f"panel {case} rows [{r0}, {r1})")
+            c, st = ctx.spgemm_rows_device(dA, r0, r1, dB)
+            try:
+                got = ctx.fetch(c)
+                ref = oracle.spgemm_rows(A, B, r0, r1)
+                oracle.compare(got, ref, tol=TOL)
+                sums, asums = ctx.row_sums(c)
+                sub = hsmu.HostMatrix(r1 - r0, A.N, A.rowptr[r0:r1 + 1] - A.rowptr[r0],
+                                      A.col[A.rowptr[r0]:A.rowptr[r1]], A.val[A.rowptr[r0]:A.rowptr[r1]])
+                want = sub.to_scipy() @ rb if r1 > r0 else np.zeros(0)
+                assert np.allclose(sums, want, rtol=0, atol=1e-12 * (1 + np.abs(asums).max(initial=0)))
+            finally:
+                ctx.free_result(c)
+    finally:
+        dA.free()
+        dB.free()
+
+
 def test_rowptr64(ctx):
     A = hsmu.gen_banded(5000, 20, 60, 300, seed=2)
     Cm, _, _ = check(ctx, A, A, "int64 row pointers", rowptr64=1)
