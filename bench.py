@@ -13,7 +13,10 @@ per a_ij + one B col/val per product + C row pointer + C col/val, over its rows)
 mean duration from HIP events on the library stream.  traffic: HBM bytes per launch from
 the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present.
 cpu_baseline: the CPU oracle (row-wise Gustavson, C + OpenMP) on the same workload,
-repeated for about 10 s, median.
+repeated for about 10 s, median.  Options come from the autotuner table for the
+workload's shape class when the table holds it (hsmu-spgemm_amd/tuning/gfx950.tsv).
+Also reported: the compensated-FP32 form, the 512-instance batched form, and rocSPARSE's
+SpGEMM on the same inputs.
 """
 import argparse
 import json
@@ -110,6 +113,30 @@ def bench_batched(hsmu, ctx, steps, warmup, table):
     return out
 
 
+def bench_rocsparse(hsmu, ctx, dA, products, ms_step, steps, warmup):
+    """rocSPARSE's SpGEMM (the vendor library) on the same device inputs: per call, the
+    device time of its nnz + compute stages and the host time of the whole call (which
+    adds its buffer and C allocations).  The speedup is quoted against the stages only,
+    the figure most favourable to rocSPARSE."""
+    from hsmu import check
+    for _ in range(warmup):
+        c, _, _ = check.rocsparse_spgemm(ctx, dA, dA)
+        check.free(c)
+    stages, walls = [], []
+    for _ in range(steps):
+        c, ms_s, ms_w = check.rocsparse_spgemm(ctx, dA, dA)
+        check.free(c)
+        stages.append(ms_s)
+        walls.append(ms_w)
+    ms_s, ms_w = statistics.median(stages), statistics.median(walls)
+    out = {"ms_stages": round(ms_s, 4), "ms_call": round(ms_w, 4),
+           "gflops_stages": round(2.0 * products / (ms_s * 1e6), 2),
+           "speedup_hsmu_vs_rocsparse_stages": round(ms_s / ms_step, 2),
+           "speedup_hsmu_vs_rocsparse_call": round(ms_w / ms_step, 2)}
+    log(f"rocsparse: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +146,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32c", action="store_true", help="skip the compensated-FP32 form")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-instance batched form")
+    ap.add_argument("--no-rocsparse", action="store_true", help="skip the rocSPARSE comparison")
     ap.add_argument("--table", default=TUNE_TABLE, help="autotuner table (shape class -> options)")
     ap.add_argument("--opts", default="", help="hsmu_opts overrides, e.g. products_per_lane=16,b_lanes=32")
     ap.add_argument("--copy-mib", type=int, default=1024,
@@ -143,7 +171,15 @@ def main():
         copy_gbs = 2.0 * nbytes / (ms * 1e-3) / 1e9  # read + write
         log(f"copy probe: {args.copy_mib} MiB in {ms:.3f} ms = {copy_gbs:.0f} GB/s (read+write)")
     overrides = dict(kv.split("=") for kv in args.opts.split(",") if kv)
-    opts = hsmu.default_opts(**{k: int(v) for k, v in overrides.items()})
+    tuned_from_table = False
+    if overrides or not os.path.exists(args.table):
+        opts = hsmu.default_opts(**{k: int(v) for k, v in overrides.items()})
+    else:
+        opts, tuned_from_table = ctx.autotune([dA], [dA], base=hsmu.default_opts(), table=args.table, search=False)
+    opt_keys = ("products_per_lane", "tiles_per_lane", "b_lanes", "t_lanes", "lds_block_budget", "persistent",
+                "rebuild_dirs")
+    log(f"options ({'autotuner table' if tuned_from_table else 'defaults/overrides'}): "
+        f"{ {k: getattr(opts, k) for k in opt_keys} }")
 
     for _ in range(args.warmup):
         c, st = ctx.spgemm_device(dA, dA, opts)
@@ -205,6 +241,10 @@ def main():
         log(f"fp32 compensated: {fp32c}")
         dA32.free()
 
+    rocsparse = None
+    if not args.no_rocsparse:
+        rocsparse = bench_rocsparse(hsmu, ctx, dA, products, ms_step, args.steps, args.warmup)
+
     batched = None
     if not args.no_batched:
         batched = bench_batched(hsmu, ctx, max(3, min(args.steps, 5)), 1, args.table)
@@ -243,7 +283,8 @@ def main():
         "config": {"workload": "banded FEM-like C=A*A: 62451 rows, nnz/row U[40,88], columns within +-400, "
                                "N(0,1) values, seed 1",
                    "rows": A.M, "nnz_a": A.nnz, "products": products, "nnz_c": nnz_c,
-                   "end_to_end_alg_GBps": round(e2e_gbs, 1), "stage_ms": stage},
+                   "end_to_end_alg_GBps": round(e2e_gbs, 1), "stage_ms": stage,
+                   "opts": {k: getattr(opts, k) for k in opt_keys}, "tuned_from_table": bool(tuned_from_table)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": f"k_numeric_lds bin {dom}" if dom < 8 else "k_numeric_global",
@@ -251,6 +292,7 @@ def main():
                      "copy_kernel_GBps": round(copy_gbs, 1) if copy_gbs else None},
         "cpu_baseline": cpu,
         "forms": {"fp32_compensated": fp32c, "batched": batched},
+        "rocsparse": rocsparse,
     }
     print(json.dumps(out), flush=True)
     dA.free()

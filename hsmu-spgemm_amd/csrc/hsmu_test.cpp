@@ -2,13 +2,15 @@
 // (src/test.cu:10-94): load A, form B = A (or A^T with --aat, the reference's AAT flag),
 // upload, run NHC_spgemm, print the reference's lines, release everything.
 //
-//   hsmu_test <matrix.mtx | gen:SPEC> [--aat] [--csv-dir DIR]
+//   hsmu_test <matrix.mtx | gen:SPEC> [--aat] [--csv-dir DIR] [--check]
 //
 // gen:SPEC names a synthetic input instead of a file:
 //   gen:laplace2d:N   gen:laplace3d:N   gen:banded:M[:SEED]   gen:rmat:SCALE[:EF[:SEED]]
 //   gen:zipf:M:N[:SEED]
 // --csv-dir DIR makes NHC_spgemm append the reference's three result CSVs
 // (src/compressed/compressed_NHC_spgemm.cu:220-246) under DIR.
+// --check runs the rocSPARSE cross-check after NHC_spgemm (the reference's CHECK_RESULT
+// build, src/test.cu:73-76 -> tile_cusparse_spgemm); the exit status is 1 on a mismatch.
 
 #include <sys/time.h>
 
@@ -20,6 +22,7 @@
 
 #include "../../include/NHC_spgemm.h"
 #include "../../include/hsmu.h"
+#include "../../include/hsmu_check.h"
 
 namespace {
 
@@ -82,13 +85,15 @@ int main(int argc, char **argv)
         return 0;
     }
     std::string spec = argv[1];
-    bool aat = false;
+    bool aat = false, check = false;
     std::string csv_dir;
     for (int i = 2; i < argc; ++i) {
         if (!strcmp(argv[i], "--aat"))
             aat = true;
         else if (!strcmp(argv[i], "--csv-dir") && i + 1 < argc)
             csv_dir = argv[++i];
+        else if (!strcmp(argv[i], "--check"))
+            check = true;
     }
     printf("the matrix is: -------------- %s --------------\n", spec.c_str());
     if (spec.rfind("gen:", 0) != 0) {
@@ -147,6 +152,9 @@ int main(int argc, char **argv)
         setenv("HSMU_CSV_DIR", csv_dir.c_str(), 1);
     NHC_spgemm(&cbin, &c_bin, &A, &B, &C, &mask);
     const int status = NHC_last_status();
+    int check_rc = 0;
+    if (check && status == HSMU_OK)
+        check_rc = tile_rocsparse_spgemm(&A, &B, nullptr, &C);
 
     realse_NHCcsr(&A);
     realse_NHCcsr(&B);
@@ -159,5 +167,7 @@ int main(int argc, char **argv)
         realse_compressed_bin(&cbin);
     }
     free_bin(&c_bin);
-    return status == HSMU_OK ? 0 : 2;
+    if (status != HSMU_OK)
+        return 2;
+    return check_rc == 0 ? 0 : (check_rc == 1 ? 1 : 3);
 }

@@ -32,6 +32,17 @@ def test_every_declared_symbol_is_exported():
     assert not missing, missing
 
 
+def test_check_library_exports_its_header():
+    """libhsmu_check.so (the rocSPARSE cross-check) loads without a GPU and exports every
+    function include/hsmu_check.h declares."""
+    from hsmu import check
+    names = declared_functions(os.path.join(ROOT, "include", "hsmu_check.h"))
+    assert "hsmu_rocsparse_spgemm" in names and "tile_rocsparse_spgemm" in names
+    missing = [n for n in names if not hasattr(check.lib, n)]
+    assert not missing, missing
+    assert ctypes.sizeof(check.Diff) == 56
+
+
 def test_struct_layouts_match_c():
     """ctypes mirrors of the ABI structs have the C sizes (compiled with gcc here)."""
     src = r"""
