@@ -20,6 +20,8 @@
 
 #include <climits>
 
+#include <type_traits>
+
 #include "device_common.h"
 #include "hsmu_internal.h"
 
@@ -990,6 +992,405 @@ __global__ __launch_bounds__(1024) void k_numeric_wcompact(const int *__restrict
     }
 }
 
+// Work items of the split wide-row kernel, one thread per row of the global bin: the row's
+// entries in runs of wide_split(nnz(C_i)), appended in any order.
+__global__ void k_wide_items(const int *__restrict__ rows, const int *__restrict__ count_p, const int *__restrict__ Arp,
+                             const long long *__restrict__ crp, WideItem *__restrict__ items, long long cap,
+                             unsigned *ctr, int *err)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= *count_p)
+        return;
+    const int row = rows[idx];
+    const int nA = Arp[row + 1] - Arp[row];
+    const long long n = crp[row + 1] - crp[row];
+    const int sz = wide_split(n);
+    const int ns = nA > 0 ? (nA + sz - 1) / sz : 1;
+    const unsigned base = atomicAdd(&ctr[0], (unsigned)ns);
+    if ((long long)base + ns > cap) { // cannot happen: cap bounds count + nnz(A) / WS_MIN_SPLIT
+        atomicOr(err, ERR_INTERNAL);
+        return;
+    }
+    for (int q = 0; q < ns; ++q) {
+        const int e0 = q * sz;
+        const int e1 = e0 + sz < nA ? e0 + sz : nA;
+        items[base + q] = WideItem{row, e0, e1, ns};
+    }
+}
+
+// Zero C's values of the rows that are split into several items (one workgroup per row).
+__global__ __launch_bounds__(256) void k_wide_zero(const int *__restrict__ rows, const int *__restrict__ count_p,
+                                                   const int *__restrict__ Arp, const long long *__restrict__ crp,
+                                                   double *__restrict__ Cval)
+{
+    const int nrows = *count_p;
+    for (int idx = blockIdx.x; idx < nrows; idx += gridDim.x) {
+        const int row = rows[idx];
+        const int nA = Arp[row + 1] - Arp[row];
+        const long long cb = crp[row], n = crp[row + 1] - cb;
+        if (nA <= wide_split(n))
+            continue;
+        for (long long q = threadIdx.x; q < n; q += 256)
+            Cval[cb + q] = 0.0;
+    }
+}
+
+// Wide rows (FP64, directory kept): the compact windowed accumulator of k_numeric_wcompact,
+// on work items pulled from a queue.  An item holds at most WS_SPLIT entries of one row, so
+// their cursors into the B rows live in LDS.  Windows are runs of directory entries that
+// share both k / WS_TILES and (slot of k) / (WS_VALS - 64), so a window holds at most
+// WS_TILES entries and WS_VALS slots; their starts are listed in parallel (a flag per
+// entry, a workgroup scan), WS_WIN at a time.  In a window, the elements of the active
+// B rows are spread over the whole workgroup: each active entry gets chunks of WS_CH
+// elements from its cursor, as many as its density (elements left / columns left to its
+// last column) predicts for the window, and every group of 16 lanes takes chunks from
+// that list.  B rows are sorted, so a chunk's in-window elements are a prefix of it; the
+// smallest (position, column) after a prefix, kept by a 64-bit LDS atomicMin, is the
+// entry's new cursor and the column there.  An entry whose chunks were all inside the
+// window gets twice as many chunks in another round.  A two-level row's coarse words are staged
+// in LDS when there are at most WS_COARSE of them.  Items of a split row add their window
+// sums into C (zeroed by k_wide_zero) with atomics; the first item writes the columns.
+constexpr int WS_COARSE = 256;
+constexpr int WS_BS = 768;   // lanes of a workgroup (two fit a CU: 6 waves per SIMD)
+constexpr int WS_WIN = 256;  // window starts listed at a time
+constexpr int WS_CH = 64;    // elements of a chunk (16 lanes x 4)
+constexpr int WS_TCAP = 1024; // chunks listed per round
+constexpr int WS_LDS = (WS_VALS * 8 + WS_TILES * 8 + WS_SPLIT * 8 /* av */ + WS_SPLIT * 8 /* pos:col */ +
+                        WS_COARSE * 8 + WS_SPLIT * 4 * 4 /* cur kend last act */ + WS_COARSE * 4 +
+                        (WS_WIN + 1) * 4 + WS_TCAP * 4 + WS_TILES * 2 + WS_SPLIT * 2 + 15) & ~15;
+constexpr u64 WS_NOPOS = 0xffffffffull << 32; // (position, column) with no position
+static_assert(WS_TILES <= 3 * WS_BS, "the column output takes three tiles per lane");
+static_assert(WS_VALS / 5 <= WS_TCAP, "tiles with more than 4 columns of a window fit task[]");
+
+__global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void k_numeric_wsplit(
+    const WideItem *__restrict__ items, const unsigned *__restrict__ ctr_listed, unsigned *ctr_taken, Operands op,
+    const long long *__restrict__ crp, int *__restrict__ Ccol, double *__restrict__ Cval, int *ws,
+    long long ft_stride, long long sym_row_max, int sym_all_global, int long_rows, int *err)
+{
+    const double *__restrict__ Aval = (const double *)op.Aval;
+    const double *__restrict__ Bval = (const double *)op.Bval;
+    extern __shared__ __align__(16) unsigned char lds[];
+    double *vals = (double *)lds;
+    u64 *wm = (u64 *)(vals + WS_VALS);
+    double *av = (double *)(wm + WS_TILES);
+    u64 *pc = (u64 *)(av + WS_SPLIT); // (position of the next element << 32) | its column
+    u64 *cm = pc + WS_SPLIT;
+    int *cur = (int *)(cm + WS_COARSE);
+    int *kend = cur + WS_SPLIT;
+    int *last = kend + WS_SPLIT; // last column of B_j
+    int *act = last + WS_SPLIT;
+    unsigned *co = (unsigned *)(act + WS_SPLIT);
+    unsigned *wst = co + WS_COARSE;
+    unsigned *task = wst + WS_WIN + 1;
+    unsigned short *wo = (unsigned short *)(task + WS_TCAP);
+    unsigned short *nch = wo + WS_TILES; // chunks listed for an entry in this round
+    __shared__ int item_s, nact_s;
+    __shared__ unsigned red[16];
+    const int l = threadIdx.x, lane = l & 63, w = l >> 6;
+    const int ll = l & 15, lg0 = lane & ~15;
+    int *ft = ws + (size_t)blockIdx.x * ft_stride; // tile of every fine entry of a two-level row
+    const unsigned nitems = *ctr_listed;
+    while (true) {
+        if (l == 0)
+            item_s = (int)atomicAdd(ctr_taken, 1u);
+        __syncthreads();
+        // item and row scalars are uniform: keep them in scalar registers
+        const unsigned it = (unsigned)uniform(item_s);
+        if (it >= nitems)
+            break;
+        const int row = uniform(items[it].row), ie0 = uniform(items[it].e0);
+        const int ne = uniform(items[it].e1) - ie0, nsplit = uniform(items[it].nsplit);
+        const int a0 = uniform(op.Arp[row]), nA = uniform(op.Arp[row + 1]) - a0;
+        const long long cb = uniform64(crp[row]);
+        const long long n = uniform64(crp[row + 1]) - cb;
+        const int mn = uniform(op.rmin[row]);
+        const int S = uniform(op.rmax[row]) - mn + 1;
+        const int tp = uniform(op.rtp[row]);
+        const long long rd = uniform64(op.rdir[row]);
+        const bool two = use_sparse(S, tp) && !sym_all_global && sym_need(S, nA, tp, long_rows) <= sym_row_max;
+        const int W = two ? (S + 63) >> 6 : 0;
+        const u64 *cmg = op.dir_mask + rd;
+        const unsigned *cog = op.dir_off + rd;
+        const u64 *fm = op.dir_mask + rd + W;
+        const unsigned *foff = op.dir_off + rd + W;
+        const bool clds = W <= WS_COARSE;
+        for (int q = l; q < W && clds; q += WS_BS) {
+            cm[q] = cmg[q];
+            co[q] = cog[q];
+        }
+        const int F = two ? (int)(uniform(cog[W - 1]) + (unsigned)__popcll(uniform64(cmg[W - 1]))) : S;
+        const unsigned off0 = F > 0 ? uniform(foff[0]) : 0u;
+        if (two) {
+            for (int q = l; q < W; q += WS_BS) {
+                u64 m = cmg[q];
+                unsigned k = cog[q];
+                while (m) {
+                    ft[k++] = (q << 6) + __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                }
+            }
+        }
+        for (int e = l; e < ne; e += WS_BS) {
+            const int j = op.Acol[a0 + ie0 + e];
+            const int k0 = op.Brp[j], k1 = op.Brp[j + 1];
+            cur[e] = k0;
+            kend[e] = k1;
+            const int c0 = k0 < k1 ? op.Bcol[k0] : INT_MAX;
+            last[e] = k0 < k1 ? op.Bcol[k1 - 1] : 0;
+            pc[e] = WS_NOPOS | (unsigned)c0;
+            av[e] = Aval[a0 + ie0 + e];
+        }
+        block_sync_global();
+        const bool split = nsplit > 1, first = ie0 == 0;
+        long long done = 0;
+        int kb = 0;         // first entry whose window start is not listed yet
+        int nw = 0, wi = 0; // windows listed, current window
+        while (true) {
+            if (wi == nw) { // list the next window starts (WS_WIN at a time, in order)
+                if (kb >= F)
+                    break;
+                int cnt_st = 0; // starts listed so far
+                for (int kk = kb; kk < F && cnt_st <= WS_WIN; kk += WS_BS) {
+                    const int k = kk + l;
+                    bool st = false;
+                    if (k < F) {
+                        if (k == kb || (k % WS_TILES) == 0) {
+                            st = true;
+                        } else {
+                            const unsigned b1 = (foff[k] - off0) / (unsigned)(WS_VALS - 64);
+                            const unsigned b0 = (foff[k - 1] - off0) / (unsigned)(WS_VALS - 64);
+                            st = b1 != b0;
+                        }
+                    }
+                    unsigned tot;
+                    const unsigned pos = group_excl_scan<WS_BS>(st ? 1u : 0u, red, tot);
+                    if (st && cnt_st + (int)pos <= WS_WIN)
+                        wst[cnt_st + pos] = (unsigned)k;
+                    cnt_st += (int)uniform(tot);
+                    __syncthreads(); // red is reused by the next scan
+                }
+                // windows: starts 0..m-1 complete; the last listed start (or F) ends them
+                int m;
+                if (cnt_st > WS_WIN) {
+                    m = WS_WIN;
+                    kb = (int)uniform(wst[WS_WIN]);
+                } else {
+                    m = cnt_st;
+                    if (l == 0)
+                        wst[m] = (unsigned)F;
+                    kb = F;
+                }
+                nw = m;
+                wi = 0;
+                __syncthreads();
+                if (nw == 0)
+                    break;
+            }
+            const int k0 = (int)uniform(wst[wi]), k1 = (int)uniform(wst[wi + 1]);
+            ++wi;
+            const int nk = k1 - k0;
+            const unsigned base = uniform(foff[k0]);
+            const int cnt = (int)((k1 == F ? n : (long long)uniform(foff[k1])) - base);
+            if (l == 0)
+                nact_s = 0;
+            for (int q = l; q < nk; q += WS_BS) {
+                wm[q] = fm[k0 + q];
+                wo[q] = (unsigned short)(foff[k0 + q] - base);
+            }
+            for (int q = l; q < cnt; q += WS_BS)
+                vals[q] = 0.0;
+            const int lastt = two ? uniform(ft[k1 - 1]) : k1 - 1;
+            const int chi = (mn + lastt + 1) << 6; // end of the window's last tile
+            __syncthreads();
+            { // the entries with elements in this window
+                const bool live = l < ne && (int)(unsigned)pc[l] < chi;
+                const u64 ms = __ballot(live);
+                int s0 = 0;
+                if (lane == 0 && ms)
+                    s0 = atomicAdd(&nact_s, __popcll(ms));
+                s0 = __shfl(s0, 0, 64);
+                if (live)
+                    act[s0 + mbcnt(ms)] = l;
+            }
+            __syncthreads();
+            int nact = uniform(nact_s);
+            auto add_c = [&](int c, double v, double a) {
+                const int tt = (c >> 6) - mn;
+                int r = tt - k0;
+                if (two) {
+                    const int q = tt >> 6;
+                    const u64 m = clds ? cm[q] : cmg[q];
+                    const unsigned o = clds ? co[q] : cog[q];
+                    r = (int)(o + (unsigned)__popcll(m & below(tt & 63))) - k0;
+                }
+                atomicAdd(&vals[(unsigned)wo[r] + (unsigned)__popcll(wm[r] & below(c & 63))], a * v);
+            };
+            for (int round = 0; nact > 0; ++round) {
+                // chunks of every active entry, as many as its density predicts for the window
+                // (doubled in every further round: the density was too low for this window)
+                int e = -1, want = 0;
+                if (l < nact) {
+                    e = act[l];
+                    const int c0 = (int)(unsigned)pc[e];
+                    const long long rem = kend[e] - cur[e];
+                    long long est = rem;
+                    if (last[e] >= chi) { // the row goes past the window: scale by its columns
+                        const long long span = (long long)last[e] - c0 + 1;
+                        est = rem * (long long)(chi - c0) / span;
+                        est = ((est + est / 4 + 8) << (round < 8 ? round : 8));
+                        est = est < rem ? est : rem;
+                    }
+                    want = (int)((est + WS_CH - 1) / WS_CH);
+                    want = want < 1 ? 1 : want;
+                }
+                unsigned tot;
+                const unsigned off = group_excl_scan<WS_BS>((unsigned)want, red, tot);
+                if (l < nact) {
+                    const int got = off >= (unsigned)WS_TCAP ? 0
+                                    : (off + want > (unsigned)WS_TCAP ? WS_TCAP - (int)off : want);
+                    nch[e] = (unsigned short)got;
+                    for (int c = 0; c < got; ++c)
+                        task[off + c] = (unsigned)e | ((unsigned)c << 9);
+                }
+                const int ntask = (int)uniform(tot) < WS_TCAP ? (int)uniform(tot) : WS_TCAP;
+                __syncthreads();
+                for (int t = l >> 4; t < ntask; t += WS_BS / 16) {
+                    const unsigned tk = task[t];
+                    const int te = (int)(tk & 511u), tc = (int)(tk >> 9);
+                    const int k = cur[te] + tc * WS_CH;
+                    const int ke = kend[te];
+                    const double a = av[te];
+                    int c[4];
+                    double v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int p = k + ll + 16 * u;
+                        c[u] = INT_MAX;
+                        v[u] = 0.0;
+                        if (p < ke) {
+                            c[u] = op.Bcol[p];
+                            v[u] = Bval[p];
+                        }
+                    }
+                    int took = 0;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const bool in = c[u] < chi;
+                        if (in)
+                            add_c(c[u], v[u], a);
+                        took += __popcll((__ballot(in) >> lg0) & 0xffffull);
+                    }
+                    if (took < WS_CH) { // the prefix ends here: offer (position, column) after it
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (ll + 16 * u == took)
+                                atomicMin((unsigned long long *)&pc[te],
+                                          ((u64)(unsigned)(k + took) << 32) | (unsigned)c[u]);
+                    }
+                }
+                __syncthreads();
+                // move the cursors; entries whose chunks were all inside the window go again
+                bool again = false;
+                if (l < nact) {
+                    const u64 r = pc[e];
+                    if ((r >> 32) != 0xffffffffull) {
+                        cur[e] = (int)(r >> 32);
+                        pc[e] = WS_NOPOS | (r & 0xffffffffull);
+                    } else {
+                        const int kn = cur[e] + (int)nch[e] * WS_CH;
+                        if (kn >= kend[e]) {
+                            cur[e] = kend[e];
+                            pc[e] = WS_NOPOS | (unsigned)INT_MAX;
+                        } else {
+                            cur[e] = kn;
+                            again = true;
+                        }
+                    }
+                }
+                if (l == 0)
+                    nact_s = 0;
+                __syncthreads();
+                const u64 ma = __ballot(again);
+                int s0 = 0;
+                if (lane == 0 && ma)
+                    s0 = atomicAdd(&nact_s, __popcll(ma));
+                s0 = __shfl(s0, 0, 64);
+                if (again)
+                    act[s0 + mbcnt(ma)] = e;
+                __syncthreads();
+                nact = uniform(nact_s);
+            }
+            // Output: columns first, then values.  The only loads of this phase (tile numbers
+            // of a two-level row, from ft) come before any store of a tile, and a flat row has
+            // none: a wait for a load also waits for every store issued before it.  A tile
+            // with at most 4 columns is written by one lane; denser tiles are listed (task[]
+            // is free here) and written by a wave each, one lane per column.
+            if (first) {
+                if (l == 0)
+                    nact_s = 0;
+                __syncthreads();
+                auto put_cols = [&](auto tile_of) {
+                    int tid[3];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        const int q = l + i * WS_BS;
+                        tid[i] = q < nk ? tile_of(q) : 0;
+                    }
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        const int q = l + i * WS_BS;
+                        u64 m = q < nk ? wm[q] : 0ull;
+                        const bool dense = __popcll(m) > 4;
+                        const u64 md = __ballot(dense);
+                        int d0 = 0;
+                        if (lane == 0 && md)
+                            d0 = atomicAdd(&nact_s, __popcll(md));
+                        d0 = __shfl(d0, 0, 64);
+                        if (dense) {
+                            task[d0 + mbcnt(md)] = (unsigned)q;
+                        } else if (m) {
+                            const int tile = mn + tid[i];
+                            long long at = cb + base + wo[q];
+                            while (m) {
+                                Ccol[at++] = (tile << 6) + __ffsll((unsigned long long)m) - 1;
+                                m &= m - 1;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    const int nd = uniform(nact_s);
+                    for (int t = w; t < nd; t += WS_BS / 64) {
+                        const int q = (int)task[t];
+                        const u64 m = wm[q];
+                        const int tile = mn + tile_of(q);
+                        if ((m >> lane) & 1ull)
+                            Ccol[cb + base + wo[q] + __popcll(m & below(lane))] = (tile << 6) + lane;
+                    }
+                };
+                if (two)
+                    put_cols([&](int q) { return ft[k0 + q]; });
+                else
+                    put_cols([&](int q) { return k0 + q; });
+            }
+            if (split) {
+                for (int q = l; q < cnt; q += WS_BS) {
+                    const double v = vals[q];
+                    if (v != 0.0)
+                        unsafeAtomicAdd(&Cval[cb + base + q], v);
+                }
+            } else {
+                for (int q = l; q < cnt; q += WS_BS)
+                    Cval[cb + base + q] = vals[q];
+            }
+            done += cnt;
+            __syncthreads();
+        }
+        if (l == 0 && done != n)
+            atomicOr(err, ERR_INTERNAL);
+        block_sync_global(); // ft and the LDS state of the next item
+    }
+}
+
 // ---- launchers -------------------------------------------------------------------------------
 
 // Grid of an LDS row kernel: one workgroup per row group, or, with a work queue, as many
@@ -1100,7 +1501,18 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
 {
     if (bin == BIN_GLOBAL) {
         const int blocks = count < wide.blocks ? count : wide.blocks;
-        if (op.dir_mask)
+        if (op.dir_mask && std::is_same<P, PrecF64>::value && wide.items) {
+            hipError_t e = launch_fill_zero(wide.ctr, 2 * sizeof(unsigned), s);
+            if (e != hipSuccess)
+                return e;
+            hipLaunchKernelGGL(k_wide_items, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, rows, count_dev,
+                               op.Arp, crp, wide.items, wide.items_cap, wide.ctr, err);
+            hipLaunchKernelGGL(k_wide_zero, dim3((unsigned)(count < 8192 ? count : 8192)), dim3(256), 0, s, rows,
+                               count_dev, op.Arp, crp, (double *)Cval);
+            hipLaunchKernelGGL(k_numeric_wsplit, dim3((unsigned)wide.blocks), dim3(WS_BS), WS_LDS, s, wide.items, wide.ctr,
+                               wide.ctr + 1, op, crp, Ccol, (double *)Cval, wide.ws, wide.ft_stride, wide.sym_row_max,
+                               wide.sym_all_global, wide.long_rows, err);
+        } else if (op.dir_mask)
             hipLaunchKernelGGL(k_numeric_wcompact<P>, dim3(blocks), dim3(1024), CW_LDS, s, rows, count_dev, op, crp,
                                Ccol, (typename P::V *)Cval, bshift > 6 ? 6 : bshift, wide.ws, wide.cur_stride,
                                wide.ft_stride, wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
@@ -1162,7 +1574,7 @@ hipError_t configure_kernels(int lds_max)
 #undef CFGP
 #undef CFGL
     CFG(k_numeric_window<PrecF64>) CFG(k_numeric_window<PrecF32C>)
-    CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>)
+    CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>) CFG(k_numeric_wsplit)
 #undef CFG
     return e;
 }

@@ -693,8 +693,8 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     // them -- the directories hold at most half as many entries as the tile-products that
     // rebuild them (banded: 1.3M entries for 10M tile-products; the batched Laplacians:
     // about as many, and rebuilding is 0.6 ms faster there) -- or when wide rows need them
-    // for the compact windows, and they fit an eighth of the device memory (36 GB on an
-    // MI355X).  Otherwise the numeric LDS kernels rebuild them and wide rows take the dense
+    // for the compact windows, and they fit a quarter of the device memory (72 GB on an
+    // MI355X; R-MAT scale 20 needs 43 GB).  Otherwise the numeric LDS kernels rebuild them and wide rows take the dense
     // window kernel.  A plan's capture pass repeats the recorded decision.
     op.rdir = rdir;
     op.dir_mask = nullptr;
@@ -709,7 +709,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             tile_products += h1.sym.work[b];
         const bool worth = 2 * dir_total <= tile_products || h1.sym.count[BIN_GLOBAL] > 0 ||
                            h1.num_wide_hint > 0;
-        const unsigned long long budget = c->total_mem / 8 > (1ull << 31) ? c->total_mem / 8 : (1ull << 31);
+        const unsigned long long budget = c->total_mem / 4 > (1ull << 31) ? c->total_mem / 4 : (1ull << 31);
         keep_dirs = !opts.rebuild_dirs && worth && dir_total > 0 && dir_total * 12ull <= budget;
         if (mode.plan)
             mode.plan->dirs_kept = keep_dirs;
@@ -810,17 +810,25 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         if (cnt <= 0)
             continue;
         // wide rows: a workspace per window workgroup (two 1024-lane workgroups fit a CU's
-        // LDS) for the cursors of its row's A entries and the tiles of a two-level directory
+        // LDS) for the tiles of a two-level directory and, without split items, the cursors
+        // of its row's A entries.  FP64 rows with kept directories run as split work items
+        // (at most count + nnz(A) / WS_MIN_SPLIT of them) on a full persistent grid.
         WideRows wide = {};
         if (b == BIN_GLOBAL) {
-            wide.blocks = cnt < 2 * c->num_cus ? cnt : 2 * c->num_cus;
-            wide.cur_stride = ((long long)h2.num.max_na[b] + 63) & ~63LL;
+            const bool split_items = keep_dirs && !fp32;
+            wide.blocks = split_items || cnt > 2 * c->num_cus ? 2 * c->num_cus : cnt;
+            wide.cur_stride = split_items ? 0 : ((long long)h2.num.max_na[b] + 63) & ~63LL;
             wide.ft_stride = keep_dirs ? (((long long)h2.num.max_span[b] + 63) & ~63LL) : 0;
             wide.sym_row_max = bp_sym_row_max;
             wide.sym_all_global = opts.force_global ? 1 : 0;
             wide.long_rows = long_rows;
-            wide.ws = ar.get<int>(wide_ws_ints(wide.cur_stride, wide.ft_stride, wide.blocks));
-            if (!wide.ws)
+            wide.ws = ar.get<int>(wide_ws_ints(wide.cur_stride, wide.ft_stride, wide.blocks) + 1);
+            if (split_items) {
+                wide.items_cap = (long long)cnt + P.nnzA / WS_MIN_SPLIT + 1;
+                wide.items = ar.get<WideItem>((size_t)wide.items_cap);
+                wide.ctr = ar.get<unsigned>(2);
+            }
+            if (!wide.ws || (split_items && (!wide.items || !wide.ctr)))
                 return fail(HSMU_ERR_NOMEM, "wide-row workspace");
         }
         if (timing)

@@ -19,6 +19,18 @@ DEVI int mbcnt(u64 m)
 
 DEVI u64 below(int b) { return (1ull << b) - 1ull; }
 
+// The first active lane's value, in a scalar register: for values that are the same in
+// every lane (loaded from one address), so that no per-lane copy is kept.
+DEVI int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+DEVI unsigned uniform(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
+DEVI long long uniform64(long long v)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v & 0xffffffffLL));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+DEVI u64 uniform64(u64 v) { return (u64)uniform64((long long)v); }
+
 // Reductions and scans over the 64 lanes of a wave (every lane must take part).
 template <typename T> DEVI T wave_sum(T v)
 {

@@ -262,6 +262,26 @@ hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *s
 // compact windowed kernel runs, and needs to know which rows the symbolic phase held in
 // global memory (their directories are flat): sym_row_max (LDS bytes a row may use) and
 // sym_all_global (every row was global: force_global).
+// Wide rows with kept directories (FP64) are processed as work items: entries [e0, e1) of
+// A row `row`.  A row whose entries x windows exceed WS_VISITS is split into several items
+// (nsplit > 1); their partial sums meet in C's values by atomic adds (C zeroed first).
+struct WideItem {
+    int row, e0, e1, nsplit;
+};
+constexpr int WS_VALS = 4096;        // value slots of one window
+constexpr int WS_TILES = 2048;       // directory entries of one window
+constexpr int WS_SPLIT = 512;        // A entries of one item, at most
+constexpr int WS_MIN_SPLIT = 64;     // A entries of one item, at least (unless the row has fewer)
+constexpr long long WS_VISITS = 65536; // target of entries x windows per item
+// A entries per item for a row with nA entries and n = nnz(C_i)
+HSMU_HD inline int wide_split(long long n)
+{
+    const long long win = n / (WS_VALS - 64) + 1; // windows, about
+    long long sz = WS_VISITS / win;
+    sz = sz < WS_MIN_SPLIT ? WS_MIN_SPLIT : (sz > WS_SPLIT ? WS_SPLIT : sz);
+    return (int)(sz & ~63LL);
+}
+
 struct WideRows {
     int *ws;
     long long cur_stride, ft_stride;
@@ -269,6 +289,11 @@ struct WideRows {
     long long sym_row_max;
     int sym_all_global;
     int long_rows; // the with_long the symbolic binning used
+    // split work items (FP64 with kept directories): capacity, list, counters
+    // ctr[0] = items listed, ctr[1] = items taken
+    long long items_cap;
+    WideItem *items;
+    unsigned *ctr;
 };
 size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks);
 

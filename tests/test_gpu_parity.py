@@ -176,6 +176,34 @@ def test_long_rows_of_B(ctx, opts):
     check(ctx, A, B, f"long rows of B {opts}", **opts)
 
 
+def split_rows_pair(seed, heavy_rows, na_lo, na_hi, K, N, b_lo, b_hi):
+    """A (heavy_rows + 200 rows x K) whose first heavy_rows rows hold na_lo..na_hi entries
+    (the rest 0..30), B (K x N) with b_lo..b_hi entries per row: the first rows of C are
+    wide and split into several work items whose sums meet in C."""
+    rng = np.random.default_rng(seed)
+    rows_b = [np.sort(rng.choice(N, size=int(rng.integers(b_lo, b_hi + 1)), replace=False)).astype(np.int32)
+              for _ in range(K)]
+    rp_b = np.zeros(K + 1, np.int32)
+    rp_b[1:] = np.cumsum([len(r) for r in rows_b])
+    B = hsmu.HostMatrix(K, N, rp_b, np.concatenate(rows_b), rng.uniform(-1, 1, int(rp_b[-1])))
+    M = heavy_rows + 200
+    rows_a = [np.sort(rng.choice(K, size=int(rng.integers(na_lo, na_hi + 1)) if i < heavy_rows
+                                 else int(rng.integers(0, 31)), replace=False)).astype(np.int32) for i in range(M)]
+    rp_a = np.zeros(M + 1, np.int32)
+    rp_a[1:] = np.cumsum([len(r) for r in rows_a])
+    A = hsmu.HostMatrix(M, K, rp_a, np.concatenate(rows_a), rng.uniform(-1, 1, int(rp_a[-1])))
+    return A, B
+
+
+@pytest.mark.parametrize("shape", [(4, 2000, 3000, 3000, 300000, 20, 80), (1, 20000, 20000, 40000, 1 << 20, 5, 15),
+                                   (6, 1500, 2500, 5000, 60000, 100, 400), (3, 3000, 3000, 6000, 1 << 21, 2, 2),
+                                   (3, 2500, 2500, 6000, 1 << 23, 2, 2)])
+@pytest.mark.parametrize("opts", [dict(), dict(force_global=1)])
+def test_split_wide_rows(ctx, shape, opts):
+    A, B = split_rows_pair(5, *shape)
+    check(ctx, A, B, f"split wide rows {shape} {opts}", **opts)
+
+
 @pytest.mark.parametrize("case", ["banded", "zipf_aat"])
 def test_row_panels_and_row_sums(ctx, case):
     """hsmu_spgemm_rows over consecutive panels reproduces the oracle's rows; the device
