@@ -650,7 +650,8 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     bp.per_lane = opts.tiles_per_lane > 0 ? opts.tiles_per_lane : 4;
     const long long bp_sym_row_max = bp.lds_row_max;
     bp.force_global = opts.force_global;
-    bp.pad = 0;
+    bp.lds_cu = lds_dyn;
+    bp.min_waves = opts.min_waves > 0 ? opts.min_waves : 8;
     HIP_TRY(launch_bin(0, M, P.Arp, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, s));
     HIP_TRY(mark(c->ev[EV_ANALYSIS]));
     DevStats h1;

@@ -33,8 +33,9 @@ struct BinParams {
     long long lds_block_budget; // LDS bytes per workgroup for bins with several rows per workgroup
     long long lds_row_max;      // LDS bytes one row may use
     long long per_lane;         // work items per lane
+    long long lds_cu;           // LDS bytes of a CU
     int force_global;
-    int pad;
+    int min_waves;              // waves per CU a one-row workgroup's width should reach
 };
 
 struct BinStats {

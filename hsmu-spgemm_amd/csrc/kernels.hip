@@ -496,6 +496,14 @@ DEVI int choose_bin(long long work, long long need, const BinParams &bp)
             break;
         ++b;
     }
+    // a one-row workgroup's LDS decides how many fit a CU: widen the row's group until
+    // those workgroups hold min_waves waves, to hide the latency of the gathers from B
+    while (b < NLDS - 1) {
+        const int G = 8 << b;
+        if (G < BLOCK || need <= 0 || (long long)(G / 64) * (bp.lds_cu / need) >= bp.min_waves)
+            break;
+        ++b;
+    }
     return b;
 }
 

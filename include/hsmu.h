@@ -123,7 +123,10 @@ typedef struct hsmu_opts {
     int32_t persistent;       /* 1: the LDS row kernels run as persistent workgroups (as
                                  many as fit the device) pulling row groups from per-XCD
                                  device work queues; 0: one workgroup per row group */
-    int32_t reserved[3];
+    int32_t min_waves;        /* a row that has a workgroup to itself gets enough lanes for
+                                 this many waves per CU at the occupancy its LDS allows (at
+                                 most 1024 lanes); 0 = 8 */
+    int32_t reserved[2];
 } hsmu_opts;
 
 #define HSMU_NBINS 10 /* 8 LDS bins (8..1024 lanes per row), global fallback, wide rows */
