@@ -113,6 +113,31 @@ def bench_batched(hsmu, ctx, steps, warmup, table):
     return out
 
 
+def fp32c_errors(hsmu, ctx, A32, dA32, o32):
+    """Errors of the compensated-FP32 result against FP64 on the same (FP32-rounded)
+    inputs: the FP64 path of this library, whose results the tests hold to 1e-12 of the
+    CPU oracle.  Relative error |c32 - c64| / |c64| (max, median over entries with
+    c64 != 0) and the error over sum |a*b| (the FP64 product of |A| and |A|)."""
+    import numpy as np
+    c, _ = ctx.spgemm_device(dA32, dA32, o32)
+    got = ctx.fetch(c)
+    ctx.free_result(c)
+    A64 = hsmu.HostMatrix(A32.M, A32.N, A32.rowptr, A32.col, A32.val.astype(np.float64))
+    Aabs = hsmu.HostMatrix(A32.M, A32.N, A32.rowptr, A32.col, np.abs(A64.val))
+    ref, _ = ctx.spgemm(A64, A64)
+    mag, _ = ctx.spgemm(Aabs, Aabs)
+    if not (np.array_equal(np.asarray(got.rowptr, np.int64), np.asarray(ref.rowptr, np.int64))
+            and np.array_equal(got.col, ref.col)):
+        raise SystemExit("compensated FP32: structure differs from FP64")
+    err = np.abs(got.val.astype(np.float64) - ref.val)
+    nz = ref.val != 0
+    rel = err[nz] / np.abs(ref.val[nz])
+    return {"rel_err_max": float(rel.max(initial=0.0)), "rel_err_median": float(np.median(rel)) if rel.size else 0.0,
+            "frac_rel_err_above_1e-6": float(np.mean(rel > 1e-6)) if rel.size else 0.0,
+            "err_over_sum_abs_max": float((err / np.maximum(mag.val, 1e-300)).max(initial=0.0)),
+            "reference": "FP64 path on the same FP32-rounded inputs"}
+
+
 def bench_rocsparse(hsmu, ctx, dA, products, ms_step, steps, warmup):
     """rocSPARSE's SpGEMM (the vendor library) on the same device inputs: per call, the
     device time of its nnz + compute stages and the host time of the whole call (which
@@ -238,6 +263,7 @@ def main():
         ms32 = (time.perf_counter() - t1) * 1e3 / args.steps
         fp32c = {"value": round(2.0 * products / (ms32 * 1e6), 3), "unit": "GFLOP/s", "ms_per_step": round(ms32, 4),
                  "speedup_vs_fp64": round(ms_step / ms32, 3), "numeric_ms": round(st32["ms_form_cval"], 4)}
+        fp32c.update(fp32c_errors(hsmu, ctx, A32, dA32, o32))
         log(f"fp32 compensated: {fp32c}")
         dA32.free()
 

@@ -155,13 +155,14 @@ DEVI unsigned scan_popcounts(const u64 *m, int n, unsigned *redu, unsigned &tot,
 // A slot is 8 bytes in both forms, so the LDS layout and the bins are the same.
 // PrecF64 (plain): FP64 values, slot = one double, ds_add_f64 per product.
 // PrecF32C (compensated FP32): FP32 values; the product is split exactly into
-// p = fl(a*b) and e = fma(a, b, -p) (TwoProduct); the slot holds a float pair (s, c):
-// p is added to s by a compare-and-swap loop (so the stored sum is the VALU's round-to-
-// nearest-even fl(s + p)), the exact rounding error of that addition is recomputed
-// (TwoSum, Knuth) and err + e go into c with a plain float atomic add.  The result is
-// fl(s + c): c carries every rounding error of the products and of the additions into
-// s, so the sum is accurate to about one FP32 rounding of the result plus O(n u^2) of
-// sum |a*b| (u = 2^-24).
+// p = fl(a*b) and e = fma(a, b, -p) (TwoProduct); the slot holds a float pair (s, c),
+// updated by one 64-bit compare-and-swap: s becomes the VALU's round-to-nearest-even
+// fl(s + p), and the exact rounding error of that addition (TwoSum, Knuth) plus e goes
+// into c.  The result is fl(s + c): c carries every rounding error of the products and
+// of the additions into s, so the sum is accurate to about one FP32 rounding of the
+// result plus O(n u^2) of sum |a*b| (u = 2^-24).  (Memory-side float atomic adds return
+// old values that are not a serial history of the stored sums when lanes collide, so
+// they cannot feed TwoSum.)
 struct PrecF64 {
     typedef double V;
     static DEVI void add(double *vals, unsigned slot, double a, V b) { atomicAdd(&vals[slot], a * b); }
@@ -170,35 +171,32 @@ struct PrecF64 {
 
 struct PrecF32C {
     typedef float V;
-    // s += p atomically with the sum rounded by the VALU (round to nearest even): a
-    // compare-and-swap loop, because the rounding of the memory units' float atomic add
-    // does not match, and the TwoSum error below must describe the addition actually
-    // stored.  The error of that addition plus e then goes into c, whose own rounding is
-    // second order (|c| ~ u |s|).
-    static DEVI void two_add(float *s, float *c, float p, float e)
-    {
-        unsigned *si = (unsigned *)s;
-        float old = *s;
-        float sum;
-        while (true) {
-            sum = old + p;
-            const unsigned want = __float_as_uint(old);
-            const unsigned prev = atomicCAS(si, want, __float_as_uint(sum));
-            if (prev == want)
-                break;
-            old = __uint_as_float(prev);
-        }
-        const float bb = sum - old;
-        const float err = (old - (sum - bb)) + (p - bb);
-        atomicAdd(c, err + e);
-    }
+    // (s, c) += p with one 64-bit compare-and-swap on the pair: the VALU forms
+    // t = fl(s + p) (round to nearest even), recovers that addition's exact error
+    // (TwoSum, Knuth) and adds it and e into c; the swap succeeds only if no other lane
+    // changed the pair meanwhile, so the error always describes the addition stored.
+    // c's own rounding is second order (|c| ~ u |s|).
     static DEVI void add(double *vals, unsigned slot, double a, V b)
     {
         const float af = (float)a;
-        const float pr = __fmul_rn(af, b);
-        const float e = __builtin_fmaf(af, b, -pr);
-        float *f = (float *)vals + 2 * slot;
-        two_add(f, f + 1, pr, e);
+        const float p = __fmul_rn(af, b);
+        const float e = __builtin_fmaf(af, b, -p);
+        unsigned long long *w = (unsigned long long *)vals + slot;
+        unsigned long long old = *w;
+        while (true) {
+            const float sv = __uint_as_float((unsigned)old);
+            const float cv = __uint_as_float((unsigned)(old >> 32));
+            const float t = sv + p;
+            const float bb = t - sv;
+            const float err = (sv - (t - bb)) + (p - bb);
+            const float c2 = cv + (err + e);
+            const unsigned long long nw =
+                (unsigned long long)__float_as_uint(t) | ((unsigned long long)__float_as_uint(c2) << 32);
+            const unsigned long long prev = atomicCAS(w, old, nw);
+            if (prev == old)
+                break;
+            old = prev;
+        }
     }
     static DEVI V result(const double *vals, unsigned slot)
     {
