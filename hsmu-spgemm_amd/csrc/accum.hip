@@ -307,6 +307,13 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
             nA = op.Arp[row + 1] - a0;
             tp = op.rtp[row];
         }
+        if constexpr (G >= 64) { // one row per workgroup: its scalars in scalar registers
+            S = uniform(S);
+            mn = uniform(mn);
+            a0 = uniform(a0);
+            nA = uniform(nA);
+            tp = uniform(tp);
+        }
         const bool sparse = use_sparse(S, tp);
         const int W = sparse ? (S + 63) >> 6 : S; // entries of the first level
         const RowLayout L = row_layout(0, S, 0, nA, tp, LONG);
@@ -476,6 +483,15 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
             a0 = op.Arp[row];
             nA = op.Arp[row + 1] - a0;
             tp = op.rtp[row];
+        }
+        if constexpr (G >= 64) { // one row per workgroup: its scalars in scalar registers
+            S = uniform(S);
+            mn = uniform(mn);
+            n = uniform(n);
+            a0 = uniform(a0);
+            nA = uniform(nA);
+            tp = uniform(tp);
+            cb = uniform64(cb);
         }
         const bool sparse = use_sparse(S, tp);
         const int W = sparse ? (S + 63) >> 6 : S; // entries of the first level
