@@ -1070,7 +1070,7 @@ constexpr int WS_WIN = 256;  // window starts listed at a time
 constexpr int WS_CH = 64;    // elements of a chunk (16 lanes x 4)
 constexpr int WS_TCAP = 1024; // chunks listed per round
 constexpr int WS_LDS = (WS_VALS * 8 + WS_TILES * 8 + WS_SPLIT * 8 /* av */ + WS_SPLIT * 8 /* pos:col */ +
-                        WS_COARSE * 8 + WS_SPLIT * 4 * 4 /* cur kend last act */ + WS_COARSE * 4 +
+                        WS_COARSE * 8 + WS_SPLIT * 4 * 5 /* cur kend last, two act lists */ + WS_COARSE * 4 +
                         (WS_WIN + 1) * 4 + WS_TCAP * 4 + WS_TILES * 2 + WS_SPLIT * 2 + 15) & ~15;
 constexpr u64 WS_NOPOS = 0xffffffffull << 32; // (position, column) with no position
 static_assert(WS_TILES <= 3 * WS_BS, "the column output takes three tiles per lane");
@@ -1092,13 +1092,13 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
     int *cur = (int *)(cm + WS_COARSE);
     int *kend = cur + WS_SPLIT;
     int *last = kend + WS_SPLIT; // last column of B_j
-    int *act = last + WS_SPLIT;
-    unsigned *co = (unsigned *)(act + WS_SPLIT);
+    int *act = last + WS_SPLIT; // two lists (rounds alternate)
+    unsigned *co = (unsigned *)(act + 2 * WS_SPLIT);
     unsigned *wst = co + WS_COARSE;
     unsigned *task = wst + WS_WIN + 1;
     unsigned short *wo = (unsigned short *)(task + WS_TCAP);
     unsigned short *nch = wo + WS_TILES; // chunks listed for an entry in this round
-    __shared__ int item_s, nact_s;
+    __shared__ int item_s, nact_s[2], ntask_s[2];
     __shared__ unsigned red[16];
     const int l = threadIdx.x, lane = l & 63, w = l >> 6;
     const int ll = l & 15, lg0 = lane & ~15;
@@ -1205,8 +1205,10 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
             const int nk = k1 - k0;
             const unsigned base = uniform(foff[k0]);
             const int cnt = (int)((k1 == F ? n : (long long)uniform(foff[k1])) - base);
-            if (l == 0)
-                nact_s = 0;
+            if (l == 0) {
+                nact_s[0] = 0;
+                ntask_s[0] = 0;
+            }
             for (int q = l; q < nk; q += WS_BS) {
                 wm[q] = fm[k0 + q];
                 wo[q] = (unsigned short)(foff[k0 + q] - base);
@@ -1216,18 +1218,47 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
             const int lastt = two ? uniform(ft[k1 - 1]) : k1 - 1;
             const int chi = (mn + lastt + 1) << 6; // end of the window's last tile
             __syncthreads();
-            { // the entries with elements in this window
-                const bool live = l < ne && (int)(unsigned)pc[l] < chi;
-                const u64 ms = __ballot(live);
-                int s0 = 0;
-                if (lane == 0 && ms)
-                    s0 = atomicAdd(&nact_s, __popcll(ms));
-                s0 = __shfl(s0, 0, 64);
-                if (live)
-                    act[s0 + mbcnt(ms)] = l;
-            }
-            __syncthreads();
-            int nact = uniform(nact_s);
+            // Chunks of entry e (has) for round r into the lists of buffer r & 1: as many as
+            // its density predicts for the window, doubled in every further round (the
+            // density was too low for this window).  Every lane of every wave calls it: the
+            // chunk counts are laid out with one wave scan and one LDS atomic per wave.
+            auto post = [&](bool has, int e, int r) {
+                const int buf = r & 1;
+                int want = 0;
+                if (has) {
+                    const int c0 = (int)(unsigned)pc[e];
+                    const long long rem = kend[e] - cur[e];
+                    long long est = rem;
+                    if (last[e] >= chi) { // the row goes past the window: scale by its columns
+                        const long long span = (long long)last[e] - c0 + 1;
+                        est = rem * (long long)(chi - c0) / span;
+                        est = ((est + est / 4 + 8) << (r < 8 ? r : 8));
+                        est = est < rem ? est : rem;
+                    }
+                    want = (int)((est + WS_CH - 1) / WS_CH);
+                    want = want < 1 ? 1 : want;
+                }
+                const int incl = wave_incl_scan(want);
+                const int wtot = __shfl(incl, 63, 64);
+                const u64 mh = __ballot(has);
+                int tb = 0, ab = 0;
+                if (lane == 0) {
+                    if (wtot)
+                        tb = atomicAdd(&ntask_s[buf], wtot);
+                    if (mh)
+                        ab = atomicAdd(&nact_s[buf], __popcll(mh));
+                }
+                tb = __shfl(tb, 0, 64);
+                ab = __shfl(ab, 0, 64);
+                if (has) {
+                    const int off = tb + incl - want;
+                    const int got = off >= WS_TCAP ? 0 : (off + want > WS_TCAP ? WS_TCAP - off : want);
+                    nch[e] = (unsigned short)got;
+                    for (int c = 0; c < got; ++c)
+                        task[off + c] = (unsigned)e | ((unsigned)c << 9);
+                    act[buf * WS_SPLIT + ab + mbcnt(mh)] = e;
+                }
+            };
             auto add_c = [&](int c, double v, double a) {
                 const int tt = (c >> 6) - mn;
                 int r = tt - k0;
@@ -1239,35 +1270,22 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 }
                 atomicAdd(&vals[(unsigned)wo[r] + (unsigned)__popcll(wm[r] & below(c & 63))], a * v);
             };
-            for (int round = 0; nact > 0; ++round) {
-                // chunks of every active entry, as many as its density predicts for the window
-                // (doubled in every further round: the density was too low for this window)
-                int e = -1, want = 0;
-                if (l < nact) {
-                    e = act[l];
-                    const int c0 = (int)(unsigned)pc[e];
-                    const long long rem = kend[e] - cur[e];
-                    long long est = rem;
-                    if (last[e] >= chi) { // the row goes past the window: scale by its columns
-                        const long long span = (long long)last[e] - c0 + 1;
-                        est = rem * (long long)(chi - c0) / span;
-                        est = ((est + est / 4 + 8) << (round < 8 ? round : 8));
-                        est = est < rem ? est : rem;
-                    }
-                    want = (int)((est + WS_CH - 1) / WS_CH);
-                    want = want < 1 ? 1 : want;
-                }
-                unsigned tot;
-                const unsigned off = group_excl_scan<WS_BS>((unsigned)want, red, tot);
-                if (l < nact) {
-                    const int got = off >= (unsigned)WS_TCAP ? 0
-                                    : (off + want > (unsigned)WS_TCAP ? WS_TCAP - (int)off : want);
-                    nch[e] = (unsigned short)got;
-                    for (int c = 0; c < got; ++c)
-                        task[off + c] = (unsigned)e | ((unsigned)c << 9);
-                }
-                const int ntask = (int)uniform(tot) < WS_TCAP ? (int)uniform(tot) : WS_TCAP;
+            // round 0: the entries with elements in this window; then those that go again
+            bool has = l < ne && (int)(unsigned)pc[l] < chi;
+            int e = l;
+            for (int round = 0;; ++round) {
+                post(has, e, round);
                 __syncthreads();
+                const int buf = round & 1;
+                const int nact = uniform(nact_s[buf]);
+                if (nact == 0)
+                    break;
+                const int nt = uniform(ntask_s[buf]);
+                const int ntask = nt < WS_TCAP ? nt : WS_TCAP;
+                if (l == 0) { // the other buffer's counters, for the next round
+                    nact_s[buf ^ 1] = 0;
+                    ntask_s[buf ^ 1] = 0;
+                }
                 for (int t = l >> 4; t < ntask; t += WS_BS / 16) {
                     const unsigned tk = task[t];
                     const int te = (int)(tk & 511u), tc = (int)(tk >> 9);
@@ -1305,7 +1323,9 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 __syncthreads();
                 // move the cursors; entries whose chunks were all inside the window go again
                 bool again = false;
+                e = -1;
                 if (l < nact) {
+                    e = act[buf * WS_SPLIT + l];
                     const u64 r = pc[e];
                     if ((r >> 32) != 0xffffffffull) {
                         cur[e] = (int)(r >> 32);
@@ -1321,18 +1341,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                         }
                     }
                 }
-                if (l == 0)
-                    nact_s = 0;
-                __syncthreads();
-                const u64 ma = __ballot(again);
-                int s0 = 0;
-                if (lane == 0 && ma)
-                    s0 = atomicAdd(&nact_s, __popcll(ma));
-                s0 = __shfl(s0, 0, 64);
-                if (again)
-                    act[s0 + mbcnt(ma)] = e;
-                __syncthreads();
-                nact = uniform(nact_s);
+                has = again;
             }
             // Output: columns first, then values.  The only loads of this phase (tile numbers
             // of a two-level row, from ft) come before any store of a tile, and a flat row has
@@ -1341,7 +1350,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
             // is free here) and written by a wave each, one lane per column.
             if (first) {
                 if (l == 0)
-                    nact_s = 0;
+                    nact_s[0] = 0;
                 __syncthreads();
                 auto put_cols = [&](auto tile_of) {
                     int tid[3];
@@ -1358,7 +1367,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                         const u64 md = __ballot(dense);
                         int d0 = 0;
                         if (lane == 0 && md)
-                            d0 = atomicAdd(&nact_s, __popcll(md));
+                            d0 = atomicAdd(&nact_s[0], __popcll(md));
                         d0 = __shfl(d0, 0, 64);
                         if (dense) {
                             task[d0 + mbcnt(md)] = (unsigned)q;
@@ -1372,7 +1381,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                         }
                     }
                     __syncthreads();
-                    const int nd = uniform(nact_s);
+                    const int nd = uniform(nact_s[0]);
                     for (int t = w; t < nd; t += WS_BS / 64) {
                         const int q = (int)task[t];
                         const u64 m = wm[q];
