@@ -136,7 +136,7 @@ def test_cancellation_keeps_structure(ctx):
 @pytest.mark.parametrize("opts", [dict(force_global=1), dict(lds_row_max=256), dict(lds_block_budget=4096),
                                   dict(products_per_lane=1), dict(products_per_lane=512, tiles_per_lane=256),
                                   dict(b_lanes=1, t_lanes=1), dict(b_lanes=64, t_lanes=64), dict(rebuild_dirs=1),
-                                  dict(rebuild_dirs=1, products_per_lane=4)])
+                                  dict(rebuild_dirs=1, products_per_lane=4), dict(min_waves=1), dict(min_waves=64)])
 def test_bins_and_fallbacks(ctx, opts):
     A = hsmu.gen_banded(4000, 10, 120, 600, seed=7)
     check(ctx, A, A, "banded under forced bins", **opts)
