@@ -258,20 +258,37 @@ __global__ __launch_bounds__(256) void k_bhead(int64_t nnz, int64_t N, const int
         atomicOr(&st->err, bad);
 }
 
+// The bits of a run are OR-ed across the wave with shuffles (a segmented suffix OR: lane
+// i gathers [i, end of its run within the wave) in six doubling steps); a run that goes on
+// past the wave's last lane is finished serially by its head (one per wave at most).
 __global__ __launch_bounds__(256) void k_bfill(int64_t nnz, const int *__restrict__ col, const int *__restrict__ flag,
                                                const int *__restrict__ tix, int *__restrict__ tcol, u64 *__restrict__ tmask)
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (k >= nnz || !flag[k])
+    const int lane = threadIdx.x & 63;
+    const bool valid = k < nnz;
+    const int c = valid ? col[k] : 0;
+    const bool head = !valid || flag[k] != 0; // past the end counts as a head: runs stop there
+    const u64 H = __ballot(head);
+    // first head after this lane within the wave (64: none)
+    const u64 after = lane < 63 ? (H >> (lane + 1)) : 0ull;
+    const int send = after ? lane + 1 + (int)__builtin_ctzll(after) : 64;
+    u64 acc = valid ? (1ull << (c & 63)) : 0ull;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const u64 other = __shfl_down(acc, o, 64);
+        if (lane + o < send)
+            acc |= other;
+    }
+    if (!valid || !head)
         return;
-    const int c = col[k];
-    const int tc = c >> 6;
-    u64 m = 1ull << (c & 63);
-    for (int64_t q = k + 1; q < nnz && !flag[q]; ++q)
-        m |= 1ull << (col[q] & 63);
+    if (send == 64) { // the run continues in the next wave
+        for (int64_t q = k + (64 - lane); q < nnz && !flag[q]; ++q)
+            acc |= 1ull << (col[q] & 63);
+    }
     const int t = tix[k];
-    tcol[t] = tc;
-    tmask[t] = m;
+    tcol[t] = c >> 6;
+    tmask[t] = acc;
 }
 
 __global__ __launch_bounds__(256) void k_btptr(int64_t K, int64_t nnz, const int *__restrict__ rp, const int *__restrict__ tix,

@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C ABI) against the CPU oracle.
 
 Structure (row pointers, columns) must be bit-exact; FP64 values must satisfy
-|c - c_ref| <= 1e-12 * sum_j |a_ij * b_jk| (the tolerance of the north-star contract;
+|c - c_ref| <= 1e-12 * sum_j |a_ij * b_jk| (the FP64 tolerance this library promises;
 LDS/L2 atomics make the summation order nondeterministic, as in the reference).
 """
 import ctypes
