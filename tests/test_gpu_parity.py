@@ -414,6 +414,27 @@ def test_fp32_compensated(ctx, case):
     assert np.median(rel) < 1e-6
 
 
+def test_fp32_compensated_duplicate_columns(ctx):
+    """Rows of B that repeat a column (summed, as in the reference): two lanes of one
+    instruction may hit one slot.  Every product of the first case lands in one entry of
+    C, two per row of B."""
+    n = 3000
+    rng = np.random.default_rng(12)
+    a = rng.uniform(-1, 1, n).astype(np.float32).astype(np.float64)
+    b = (rng.uniform(-1, 1, 2 * n) * 10.0 ** rng.integers(-3, 4, 2 * n)).astype(np.float32).astype(np.float64)
+    A = hsmu.HostMatrix(1, n, np.array([0, n], np.int32), np.arange(n, dtype=np.int32), a)
+    B = hsmu.HostMatrix(n, 1, np.arange(0, 2 * n + 1, 2, dtype=np.int32), np.zeros(2 * n, np.int32), b)
+    rel = check_fp32c(ctx, A, B, "one entry, duplicated columns")
+    assert rel[0] < 1e-6
+    Bd = hsmu.gen_random(400, 3000, 30, 13)
+    col = np.repeat(Bd.col, 2)
+    val = np.repeat(Bd.val, 2) * 0.5
+    Bd = hsmu.HostMatrix(Bd.M, Bd.N, (2 * np.asarray(Bd.rowptr)).astype(np.int32), col.astype(np.int32),
+                         val.astype(np.float32).astype(np.float64))
+    A2 = hsmu.gen_random(500, 400, 40, 14)
+    check_fp32c(ctx, A2, Bd, "random with duplicated columns")
+
+
 @pytest.mark.parametrize("opts", [dict(force_global=1), dict(lds_row_max=512), dict(rebuild_dirs=1)])
 def test_fp32_compensated_fallbacks(ctx, opts):
     A = hsmu.gen_banded(3000, 20, 90, 700, seed=9)
