@@ -204,6 +204,55 @@ def test_split_wide_rows(ctx, shape, opts):
     check(ctx, A, B, f"split wide rows {shape} {opts}", **opts)
 
 
+def test_wide_rows_repeated_entries_and_empty_B_rows(ctx):
+    """Wide rows (split into several work items) whose row of A names one row of B several
+    times, and rows of B that are empty: repeated entries add up, empty rows add nothing."""
+    A, B = split_rows_pair(8, 2, 1500, 2500, 3000, 200000, 20, 60)
+    rp_b = np.asarray(B.rowptr, np.int64)
+    keep = np.ones(B.nnz, bool)
+    for j in range(0, B.M, 7):  # empty every 7th row of B
+        keep[rp_b[j]:rp_b[j + 1]] = False
+    lens = np.diff(rp_b).copy()
+    lens[::7] = 0
+    rp2 = np.zeros(B.M + 1, np.int32)
+    rp2[1:] = np.cumsum(lens)
+    B2 = hsmu.HostMatrix(B.M, B.N, rp2, B.col[keep], B.val[keep])
+    rows = []
+    vals = []
+    rng = np.random.default_rng(9)
+    for i in range(A.M):
+        c = A.col[A.rowptr[i]:A.rowptr[i + 1]]
+        v = A.val[A.rowptr[i]:A.rowptr[i + 1]]
+        if i < 2:  # repeat a third of the entries of the wide rows
+            extra = rng.choice(len(c), size=len(c) // 3, replace=False)
+            order = np.argsort(np.concatenate([c, c[extra]]), kind="stable")
+            c = np.concatenate([c, c[extra]])[order]
+            v = np.concatenate([v, v[extra] * 0.5])[order]
+        rows.append(c)
+        vals.append(v)
+    rp_a = np.zeros(A.M + 1, np.int32)
+    rp_a[1:] = np.cumsum([len(r) for r in rows])
+    A2 = hsmu.HostMatrix(A.M, A.N, rp_a, np.concatenate(rows).astype(np.int32), np.concatenate(vals))
+    for opts in (dict(), dict(force_global=1)):
+        check(ctx, A2, B2, f"wide rows, repeated entries, empty rows of B {opts}", **opts)
+
+
+def test_row_panel_errors(ctx):
+    A = hsmu.gen_banded(300, 5, 20, 50, seed=2)
+    dA = ctx.matrix(A)
+    try:
+        for r0, r1 in [(-1, 10), (20, 10), (0, A.M + 1)]:
+            note(f"panel error rows [{r0}, {r1})")
+            with pytest.raises(hsmu.HsmuError) as ei:
+                ctx.spgemm_rows_device(dA, r0, r1, dA)
+            assert ei.value.status == hsmu.HSMU_ERR_INVALID
+        c, _ = ctx.spgemm_rows_device(dA, 0, A.M, dA)  # the context stays usable
+        oracle.compare(ctx.fetch(c), oracle.spgemm(A, A), tol=TOL)
+        ctx.free_result(c)
+    finally:
+        dA.free()
+
+
 @pytest.mark.parametrize("case", ["banded", "zipf_aat"])
 def test_row_panels_and_row_sums(ctx, case):
     """hsmu_spgemm_rows over consecutive panels reproduces the oracle's rows; the device
