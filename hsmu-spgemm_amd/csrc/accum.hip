@@ -1,20 +1,25 @@
 // accum.hip -- symbolic and numeric row kernels of the HSMU SpGEMM path for gfx950.
 //
-// Symbolic: nnz(C_i) from the OR of B's 64-column tile masks over row i of A, in a dense
-// mask directory over the row's tile span (reference: compressed_tilePtr_symbolic.cuh
-// and compressed_tileOR.cuh, two hash passes over tiles there; one pass and no hashing
-// here).  Numeric: the HSMU accumulator -- exactly nnz(C_i) FP64 slots in LDS, the slot
-// of column c being off[t] + popcount(mask[t] & below(c & 63)), t = (c >> 6) - rmin,
-// i.e. c's rank among C's sorted columns found in O(1) (reference: binary search over
-// the sorted column list, src/compressed/compressed_Cval.cuh:5-69 and
-// src/small/numeric.cuh:233-253).  The numeric kernel also writes C's sorted columns,
-// replacing the reference's rank-sort pass (compressed_Ccol_symbolic.cuh:4-63).
+// Symbolic: nnz(C_i) from the OR of B's 64-column tile masks over row i of A, into the
+// row's tile directory in LDS (flat: one mask per tile of the row's span; two-level for
+// sparse spans: a coarse bitmap of the non-empty tiles, then their masks), with
+// ds_or_b64; a popcount and a group scan give nnz(C_i) and each tile's first slot
+// (reference: compressed_tilePtr_symbolic.cuh and compressed_tileOR.cuh, two hash
+// passes over tiles there; one pass and no hashing here).
 //
-// LDS kernels first stage the row's A entries in LDS (B row start, tile start, a_ij, and
-// prefix sums of nnz(B_j) and tiles(B_j)), then walk the row's products (or
-// tile-products) as one flat sequence: each wave takes 64 consecutive items, so lanes
-// load consecutive elements of B (coalesced) and no lane waits on a per-entry chain of
-// dependent loads.  Rows whose slice does not fit LDS use the global-memory kernels.
+// Numeric: the HSMU accumulator -- exactly nnz(C_i) slots in LDS, the slot of column c
+// being off[t] + popcount(mask[t] & below(c & 63)) for c's directory entry t, i.e. c's
+// rank among C's sorted columns found in O(1) (reference: binary search over the sorted
+// column list, src/compressed/compressed_Cval.cuh:5-69 and src/small/numeric.cuh:233-253);
+// a dense form (one slot per column of a short, full span) skips the directory.  The
+// numeric kernels also write C's sorted columns, replacing the reference's rank-sort pass
+// (compressed_Ccol_symbolic.cuh:4-63).
+//
+// LDS row kernels stage the row's A entries (B row range, first tile, a_ij) in LDS once;
+// sub-groups of lanes then take one entry each and walk B_j, several elements per lane
+// with all loads issued first; rows of B longer than LONG_SEG (tiles: LONG_TILES) are
+// walked by whole waves.  Rows whose slice does not fit LDS (wide rows) use the windowed
+// kernels at the end of this file.
 
 #include <hip/hip_runtime.h>
 
