@@ -1011,21 +1011,91 @@ __global__ __launch_bounds__(1024) void k_numeric_wcompact(const int *__restrict
     }
 }
 
-// Work items of the split wide-row kernel, one thread per row of the global bin: the row's
-// entries in runs of wide_split(nnz(C_i)), appended in any order.
-__global__ void k_wide_items(const int *__restrict__ rows, const int *__restrict__ count_p, const int *__restrict__ Arp,
-                             const long long *__restrict__ crp, WideItem *__restrict__ items, long long cap,
-                             unsigned *ctr, int *err)
+// Work items of the split wide-row kernel: each row's entries in runs of
+// wide_split(nnz(C_i)), listed largest rows first (classes of half an octave of nnz(C_i),
+// by a counting sort): the big rows start early (their items bound the tail), and the
+// workgroups that run together work on rows of similar size, which move through their
+// columns, and so through the shared rows of B, at similar paces.
+// ctr: [0] items listed, [1] items taken, [2, 2 + WS_CLASSES) items per class,
+// [2 + WS_CLASSES, 2 + 2 WS_CLASSES) first item of each class.
+constexpr int WS_CLASSES = 32;
+DEVI int wide_class(long long n)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= *count_p)
-        return;
-    const int row = rows[idx];
-    const int nA = Arp[row + 1] - Arp[row];
+    const int lg2 = n > 1 ? 2 * (63 - __builtin_clzll((unsigned long long)n)) +
+                                (((unsigned long long)n >> (62 - __builtin_clzll((unsigned long long)n))) & 1)
+                          : 0; // 2 log2(n), rounded down to half octaves
+    const int c = lg2 - 20; // below 2^10: class 0
+    return c < 0 ? 0 : (c >= WS_CLASSES ? WS_CLASSES - 1 : c);
+}
+
+DEVI void wide_row_items(const int *rows, int idx, const int *Arp, const long long *crp, int &row, int &nA, int &sz,
+                         int &ns, int &cls)
+{
+    row = rows[idx];
+    nA = Arp[row + 1] - Arp[row];
     const long long n = crp[row + 1] - crp[row];
-    const int sz = wide_split(n);
-    const int ns = nA > 0 ? (nA + sz - 1) / sz : 1;
-    const unsigned base = atomicAdd(&ctr[0], (unsigned)ns);
+    sz = wide_split(n);
+    ns = nA > 0 ? (nA + sz - 1) / sz : 1;
+    cls = wide_class(n);
+}
+
+// pass 1: items per class (per-workgroup counts in LDS, one global atomic per class)
+__global__ __launch_bounds__(256) void k_wide_count(const int *__restrict__ rows, const int *__restrict__ count_p,
+                                                    const int *__restrict__ Arp, const long long *__restrict__ crp,
+                                                    unsigned *ctr)
+{
+    __shared__ unsigned lc[WS_CLASSES];
+    if (threadIdx.x < WS_CLASSES)
+        lc[threadIdx.x] = 0;
+    __syncthreads();
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx < *count_p) {
+        int row, nA, sz, ns, cls;
+        wide_row_items(rows, idx, Arp, crp, row, nA, sz, ns, cls);
+        atomicAdd(&lc[cls], (unsigned)ns);
+    }
+    __syncthreads();
+    if (threadIdx.x < WS_CLASSES && lc[threadIdx.x])
+        atomicAdd(&ctr[2 + threadIdx.x], lc[threadIdx.x]);
+}
+
+// pass 2 (one thread): first item of each class, largest class first; the total
+__global__ void k_wide_order(unsigned *ctr)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    unsigned o = 0;
+    for (int c = WS_CLASSES - 1; c >= 0; --c) {
+        ctr[2 + WS_CLASSES + c] = o;
+        o += ctr[2 + c];
+    }
+    ctr[0] = o;
+}
+
+// pass 3: the items, at per-workgroup ranges of their class
+__global__ __launch_bounds__(256) void k_wide_fill(const int *__restrict__ rows, const int *__restrict__ count_p,
+                                                   const int *__restrict__ Arp, const long long *__restrict__ crp,
+                                                   WideItem *__restrict__ items, long long cap, unsigned *ctr, int *err)
+{
+    __shared__ unsigned lc[WS_CLASSES], lb[WS_CLASSES];
+    if (threadIdx.x < WS_CLASSES)
+        lc[threadIdx.x] = 0;
+    __syncthreads();
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    const bool have = idx < *count_p;
+    int row = 0, nA = 0, sz = 1, ns = 0, cls = 0;
+    unsigned local = 0;
+    if (have) {
+        wide_row_items(rows, idx, Arp, crp, row, nA, sz, ns, cls);
+        local = atomicAdd(&lc[cls], (unsigned)ns);
+    }
+    __syncthreads();
+    if (threadIdx.x < WS_CLASSES && lc[threadIdx.x])
+        lb[threadIdx.x] = atomicAdd(&ctr[2 + WS_CLASSES + threadIdx.x], lc[threadIdx.x]);
+    __syncthreads();
+    if (!have)
+        return;
+    const unsigned base = lb[cls] + local;
     if ((long long)base + ns > cap) { // cannot happen: cap bounds count + nnz(A) / WS_MIN_SPLIT
         atomicOr(err, ERR_INTERNAL);
         return;
@@ -1080,6 +1150,7 @@ constexpr int WS_LDS = (WS_VALS * 8 + WS_TILES * 8 + WS_SPLIT * 8 /* av */ + WS_
 constexpr u64 WS_NOPOS = 0xffffffffull << 32; // (position, column) with no position
 static_assert(WS_TILES <= 3 * WS_BS, "the column output takes three tiles per lane");
 static_assert(WS_VALS / 5 <= WS_TCAP, "tiles with more than 4 columns of a window fit task[]");
+static_assert(2 + 2 * WS_CLASSES <= WIDE_CTRS, "wide-row counters");
 
 __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void k_numeric_wsplit(
     const WideItem *__restrict__ items, const unsigned *__restrict__ ctr_listed, unsigned *ctr_taken, Operands op,
@@ -1530,11 +1601,14 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
     if (bin == BIN_GLOBAL) {
         const int blocks = count < wide.blocks ? count : wide.blocks;
         if (op.dir_mask && std::is_same<P, PrecF64>::value && wide.items) {
-            hipError_t e = launch_fill_zero(wide.ctr, 2 * sizeof(unsigned), s);
+            hipError_t e = launch_fill_zero(wide.ctr, WIDE_CTRS * sizeof(unsigned), s);
             if (e != hipSuccess)
                 return e;
-            hipLaunchKernelGGL(k_wide_items, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, rows, count_dev,
-                               op.Arp, crp, wide.items, wide.items_cap, wide.ctr, err);
+            const unsigned gr = (unsigned)((count + 255) / 256);
+            hipLaunchKernelGGL(k_wide_count, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, wide.ctr);
+            hipLaunchKernelGGL(k_wide_order, dim3(1), dim3(64), 0, s, wide.ctr);
+            hipLaunchKernelGGL(k_wide_fill, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, wide.items,
+                               wide.items_cap, wide.ctr, err);
             hipLaunchKernelGGL(k_wide_zero, dim3((unsigned)(count < 8192 ? count : 8192)), dim3(256), 0, s, rows,
                                count_dev, op.Arp, crp, (double *)Cval);
             hipLaunchKernelGGL(k_numeric_wsplit, dim3((unsigned)wide.blocks), dim3(WS_BS), WS_LDS, s, wide.items, wide.ctr,

@@ -827,7 +827,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             if (split_items) {
                 wide.items_cap = (long long)cnt + P.nnzA / WS_MIN_SPLIT + 1;
                 wide.items = ar.get<WideItem>((size_t)wide.items_cap);
-                wide.ctr = ar.get<unsigned>(2);
+                wide.ctr = ar.get<unsigned>(WIDE_CTRS);
             }
             if (!wide.ws || (split_items && (!wide.items || !wide.ctr)))
                 return fail(HSMU_ERR_NOMEM, "wide-row workspace");
