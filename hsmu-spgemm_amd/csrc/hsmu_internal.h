@@ -269,6 +269,7 @@ hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *s
 struct WideItem {
     int row, e0, e1, nsplit;
 };
+constexpr int WIDE_CTRS = 66;
 constexpr int WS_VALS = 4096;        // value slots of one window
 constexpr int WS_TILES = 2048;       // directory entries of one window
 constexpr int WS_SPLIT = 512;        // A entries of one item, at most
@@ -290,8 +291,8 @@ struct WideRows {
     long long sym_row_max;
     int sym_all_global;
     int long_rows; // the with_long the symbolic binning used
-    // split work items (FP64 with kept directories): capacity, list, counters
-    // ctr[0] = items listed, ctr[1] = items taken
+    // split work items (FP64 with kept directories): capacity, list, WIDE_CTRS counters
+    // (ctr[0] = items listed, ctr[1] = items taken, then per size class)
     long long items_cap;
     WideItem *items;
     unsigned *ctr;
