@@ -299,6 +299,10 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
+    def trim(self) -> None:
+        """Give the context's cached device memory back to the device."""
+        _check(lib.hsmu_ctx_trim(self.handle), "hsmu_ctx_trim")
+
     @property
     def lds_max(self) -> int:
         return lib.hsmu_ctx_lds_max(self.handle)

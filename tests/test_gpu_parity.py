@@ -297,6 +297,32 @@ def test_rmat(ctx, scale):
     check(ctx, A, A, f"rmat scale {scale}")
 
 
+def test_rmat_scale16_sampled_rows(ctx):
+    """R-MAT scale 16 (nnz(C) 163.8 M, widest row 37,461 entries) computed whole, so its
+    wide rows take the split work-item kernel: the 16 widest rows and 48 seeded random rows
+    are compared with the oracle's, and every row sum with A * rowsum(B)."""
+    A = hsmu.gen_rmat(16, 16, seed=2)
+    note(f"rmat scale 16 whole product: A {A.M}x{A.N} nnz {A.nnz}")
+    dA = ctx.matrix(A)
+    try:
+        c, st = ctx.spgemm_device(dA, dA)
+        try:
+            sums, asums = ctx.row_sums(c)
+            rb = np.bincount(np.repeat(np.arange(A.M), np.diff(A.rowptr)), weights=A.val, minlength=A.M)
+            assert np.all(np.abs(sums - A.to_scipy() @ rb) <= TOL * (1 + asums))
+            got = ctx.fetch(c)
+        finally:
+            ctx.free_result(c)
+    finally:
+        dA.free()
+    lens = np.diff(got.rowptr)
+    assert st["nnz_c"] == int(got.rowptr[-1]) == 163791280
+    rows = np.concatenate([np.argsort(-lens, kind="stable")[:16],
+                           np.random.default_rng(16).integers(0, A.M, size=48)])
+    for r in (int(x) for x in rows):
+        oracle.compare(got, oracle.spgemm_rows(A, A, r, r + 1), r0=r, r1=r + 1, tol=TOL)
+
+
 def test_zipf_aat(ctx):
     A = hsmu.gen_zipf(4000, 2000, 8, 1.2, seed=4)
     check(ctx, A, hsmu.transpose(A), "zipf A*A^T")
