@@ -13,7 +13,8 @@
 // column list, src/compressed/compressed_Cval.cuh:5-69 and src/small/numeric.cuh:233-253);
 // a dense form (one slot per column of a short, full span) skips the directory.  The
 // numeric kernels also write C's sorted columns, replacing the reference's rank-sort pass
-// (compressed_Ccol_symbolic.cuh:4-63).
+// (compressed_Ccol_symbolic.cuh:4-63); for the wide rows of the FP64 path a separate
+// streaming kernel (k_wide_cols) writes them from the kept directories.
 //
 // LDS row kernels stage the row's A entries (B row range, first tile, a_ij) in LDS once;
 // sub-groups of lanes then take one entry each and walk B_j, several elements per lane
@@ -736,6 +737,13 @@ DEVI void collect_active(const int *nxt, const int *cur, const int *kend, int b0
     }
 }
 
+// The directory form the symbolic kernels stored for a row of the wide bin: two-level
+// (coarse words, then compact fine entries) or flat (one entry per tile of the span).
+DEVI bool wide_two_level(int S, int nA, int tp, long long sym_row_max, int sym_all_global, int long_rows)
+{
+    return use_sparse(S, tp) && !sym_all_global && sym_need(S, nA, tp, long_rows) <= sym_row_max;
+}
+
 template <typename P>
 __global__ __launch_bounds__(1024) void k_numeric_window(const int *__restrict__ rows, const int *__restrict__ count_p,
                                                          Operands op, const long long *__restrict__ crp,
@@ -897,7 +905,7 @@ __global__ __launch_bounds__(1024) void k_numeric_wcompact(const int *__restrict
         const int tp = op.rtp[row];
         const long long rd = op.rdir[row];
         // the form the symbolic phase kept: two-level only if the row was in an LDS bin there
-        const bool two = use_sparse(S, tp) && !sym_all_global && sym_need(S, nA, tp, long_rows) <= sym_row_max;
+        const bool two = wide_two_level(S, nA, tp, sym_row_max, sym_all_global, long_rows);
         const int W = two ? (S + 63) >> 6 : 0;
         const u64 *cmask = op.dir_mask + rd;      // coarse words (two-level)
         const unsigned *coff = op.dir_off + rd;
@@ -1107,20 +1115,87 @@ __global__ __launch_bounds__(256) void k_wide_fill(const int *__restrict__ rows,
     }
 }
 
-// Zero C's values of the rows that are split into several items (one workgroup per row).
-__global__ __launch_bounds__(256) void k_wide_zero(const int *__restrict__ rows, const int *__restrict__ count_p,
-                                                   const int *__restrict__ Arp, const long long *__restrict__ crp,
-                                                   double *__restrict__ Cval)
+constexpr int WC_BS = 256;    // lanes of a k_wide_cols workgroup
+constexpr int WC_DENSE = 1024; // dense tiles listed per pass (more are written by their lane)
+
+// Columns of the wide rows, from their kept directories, before k_numeric_wsplit adds the
+// values; and zeroed values for the rows split into several items (their items add into
+// C).  One workgroup per row.  Flat directory: a lane per tile; two-level: a lane per coarse
+// word, walking its tiles.  A tile with at most 4 columns is written by its lane; denser
+// tiles are listed in LDS and written by a wave each, one lane per column.
+__global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ rows, const int *__restrict__ count_p,
+                                                     Operands op, const long long *__restrict__ crp,
+                                                     int *__restrict__ Ccol, double *__restrict__ Cval,
+                                                     long long sym_row_max, int sym_all_global, int long_rows)
 {
+    __shared__ unsigned dk[WC_DENSE]; // dense tiles: fine entry
+    __shared__ int dtile[WC_DENSE];   // and tile (absolute)
+    __shared__ int nd_s;
+    const int l = threadIdx.x, lane = l & 63, w = l >> 6;
     const int nrows = *count_p;
     for (int idx = blockIdx.x; idx < nrows; idx += gridDim.x) {
-        const int row = rows[idx];
-        const int nA = Arp[row + 1] - Arp[row];
-        const long long cb = crp[row], n = crp[row + 1] - cb;
-        if (nA <= wide_split(n))
+        const int row = uniform(rows[idx]);
+        const int nA = uniform(op.Arp[row + 1]) - uniform(op.Arp[row]);
+        const long long cb = uniform64(crp[row]);
+        const long long n = uniform64(crp[row + 1]) - cb;
+        if (n <= 0)
             continue;
-        for (long long q = threadIdx.x; q < n; q += 256)
-            Cval[cb + q] = 0.0;
+        if (nA > wide_split(n))
+            for (long long q = l; q < n; q += WC_BS)
+                Cval[cb + q] = 0.0;
+        const int mn = uniform(op.rmin[row]);
+        const int S = uniform(op.rmax[row]) - mn + 1;
+        const bool two = wide_two_level(S, nA, uniform(op.rtp[row]), sym_row_max, sym_all_global, long_rows);
+        const int W = two ? (S + 63) >> 6 : 0;
+        const long long rd = uniform64(op.rdir[row]);
+        const u64 *cmg = op.dir_mask + rd;
+        const unsigned *cog = op.dir_off + rd;
+        const u64 *fm = op.dir_mask + rd + W;
+        const unsigned *foff = op.dir_off + rd + W;
+        auto emit = [&](int k, int tile) {
+            u64 m = fm[k];
+            if (__popcll(m) > 4) {
+                const int p = atomicAdd(&nd_s, 1);
+                if (p < WC_DENSE) {
+                    dk[p] = (unsigned)k;
+                    dtile[p] = tile;
+                    return;
+                }
+            }
+            long long at = cb + foff[k];
+            while (m) {
+                Ccol[at++] = (tile << 6) + __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+            }
+        };
+        const int U = two ? W : S; // units: coarse words, or tiles
+        for (int u0 = 0; u0 < U; u0 += WC_BS) {
+            if (l == 0)
+                nd_s = 0;
+            __syncthreads();
+            const int u = u0 + l;
+            if (u < U) {
+                if (two) {
+                    u64 cw = cmg[u];
+                    int k = (int)cog[u];
+                    while (cw) {
+                        emit(k++, mn + (u << 6) + __ffsll((unsigned long long)cw) - 1);
+                        cw &= cw - 1;
+                    }
+                } else {
+                    emit(u, mn + u);
+                }
+            }
+            __syncthreads();
+            const int nd = min(uniform(nd_s), WC_DENSE);
+            for (int t = w; t < nd; t += WC_BS / 64) {
+                const int k = (int)dk[t];
+                const u64 m = fm[k];
+                if ((m >> lane) & 1ull)
+                    Ccol[cb + foff[k] + __popcll(m & below(lane))] = (dtile[t] << 6) + lane;
+            }
+            __syncthreads(); // dk, dtile and nd_s of the next pass
+        }
     }
 }
 
@@ -1128,28 +1203,29 @@ __global__ __launch_bounds__(256) void k_wide_zero(const int *__restrict__ rows,
 // on work items pulled from a queue.  An item holds at most WS_SPLIT entries of one row, so
 // their cursors into the B rows live in LDS.  Windows are runs of directory entries that
 // share both k / WS_TILES and (slot of k) / (WS_VALS - 64), so a window holds at most
-// WS_TILES entries and WS_VALS slots; their starts are listed in parallel (a flag per
-// entry, a workgroup scan), WS_WIN at a time.  In a window, the elements of the active
+// WS_TILES entries and WS_VALS slots; their starts are listed in parallel (WS_LV entries
+// per lane, a workgroup scan), WS_WIN at a time.  In a window, the elements of the active
 // B rows are spread over the whole workgroup: each active entry gets chunks of WS_CH
 // elements from its cursor, as many as its density (elements left / columns left to its
-// last column) predicts for the window, and every group of 16 lanes takes chunks from
+// last column) predicts for the window, and every group of WS_GL lanes takes chunks from
 // that list.  B rows are sorted, so a chunk's in-window elements are a prefix of it; the
 // smallest (position, column) after a prefix, kept by a 64-bit LDS atomicMin, is the
 // entry's new cursor and the column there.  An entry whose chunks were all inside the
 // window gets twice as many chunks in another round.  A two-level row's coarse words are staged
 // in LDS when there are at most WS_COARSE of them.  Items of a split row add their window
-// sums into C (zeroed by k_wide_zero) with atomics; the first item writes the columns.
+// sums into C (zeroed by k_wide_cols, which also writes every wide row's columns) with
+// atomics; the other rows store them.
 constexpr int WS_COARSE = 256;
 constexpr int WS_BS = 768;   // lanes of a workgroup (two fit a CU: 6 waves per SIMD)
 constexpr int WS_WIN = 256;  // window starts listed at a time
-constexpr int WS_CH = 64;    // elements of a chunk (16 lanes x 4)
+constexpr int WS_LV = 8;     // directory entries per lane when listing window starts
+constexpr int WS_GL = 16;    // lanes of a chunk group
+constexpr int WS_CH = 4 * WS_GL; // elements of a chunk (WS_GL lanes x 4)
 constexpr int WS_TCAP = 1024; // chunks listed per round
 constexpr int WS_LDS = (WS_VALS * 8 + WS_TILES * 8 + WS_SPLIT * 8 /* av */ + WS_SPLIT * 8 /* pos:col */ +
                         WS_COARSE * 8 + WS_SPLIT * 4 * 5 /* cur kend last, two act lists */ + WS_COARSE * 4 +
                         (WS_WIN + 1) * 4 + WS_TCAP * 4 + WS_TILES * 2 + WS_SPLIT * 2 + 15) & ~15;
 constexpr u64 WS_NOPOS = 0xffffffffull << 32; // (position, column) with no position
-static_assert(WS_TILES <= 3 * WS_BS, "the column output takes three tiles per lane");
-static_assert(WS_VALS / 5 <= WS_TCAP, "tiles with more than 4 columns of a window fit task[]");
 static_assert(2 + 2 * WS_CLASSES <= WIDE_CTRS, "wide-row counters");
 
 __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void k_numeric_wsplit(
@@ -1176,8 +1252,8 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
     unsigned short *nch = wo + WS_TILES; // chunks listed for an entry in this round
     __shared__ int item_s, nact_s[2], ntask_s[2];
     __shared__ unsigned red[16];
-    const int l = threadIdx.x, lane = l & 63, w = l >> 6;
-    const int ll = l & 15, lg0 = lane & ~15;
+    const int l = threadIdx.x, lane = l & 63;
+    const int ll = l & (WS_GL - 1), lg0 = lane & ~(WS_GL - 1);
     int *ft = ws + (size_t)blockIdx.x * ft_stride; // tile of every fine entry of a two-level row
     const unsigned nitems = *ctr_listed;
     while (true) {
@@ -1197,7 +1273,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
         const int S = uniform(op.rmax[row]) - mn + 1;
         const int tp = uniform(op.rtp[row]);
         const long long rd = uniform64(op.rdir[row]);
-        const bool two = use_sparse(S, tp) && !sym_all_global && sym_need(S, nA, tp, long_rows) <= sym_row_max;
+        const bool two = wide_two_level(S, nA, tp, sym_row_max, sym_all_global, long_rows);
         const int W = two ? (S + 63) >> 6 : 0;
         const u64 *cmg = op.dir_mask + rd;
         const unsigned *cog = op.dir_off + rd;
@@ -1231,7 +1307,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
             av[e] = Aval[a0 + ie0 + e];
         }
         block_sync_global();
-        const bool split = nsplit > 1, first = ie0 == 0;
+        const bool split = nsplit > 1;
         long long done = 0;
         int kb = 0;         // first entry whose window start is not listed yet
         int nw = 0, wi = 0; // windows listed, current window
@@ -1240,22 +1316,38 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 if (kb >= F)
                     break;
                 int cnt_st = 0; // starts listed so far
-                for (int kk = kb; kk < F && cnt_st <= WS_WIN; kk += WS_BS) {
-                    const int k = kk + l;
-                    bool st = false;
-                    if (k < F) {
-                        if (k == kb || (k % WS_TILES) == 0) {
-                            st = true;
-                        } else {
-                            const unsigned b1 = (foff[k] - off0) / (unsigned)(WS_VALS - 64);
-                            const unsigned b0 = (foff[k - 1] - off0) / (unsigned)(WS_VALS - 64);
-                            st = b1 != b0;
+                // WS_LV consecutive entries per lane, their offsets loaded together: one
+                // round trip covers WS_BS * WS_LV entries
+                for (int kk = kb; kk < F && cnt_st <= WS_WIN; kk += WS_BS * WS_LV) {
+                    const int kl = kk + l * WS_LV;
+                    unsigned f[WS_LV + 1]; // foff[kl - 1 + i]
+#pragma unroll
+                    for (int i = 0; i <= WS_LV; ++i) {
+                        const int k = kl - 1 + i;
+                        f[i] = k >= kb && k < F ? foff[k] : 0u;
+                    }
+                    unsigned bits = 0;
+#pragma unroll
+                    for (int i = 0; i < WS_LV; ++i) {
+                        const int k = kl + i;
+                        bool st = false;
+                        if (k < F) {
+                            if (k == kb || (k % WS_TILES) == 0)
+                                st = true;
+                            else
+                                st = (f[i + 1] - off0) / (unsigned)(WS_VALS - 64) !=
+                                     (f[i] - off0) / (unsigned)(WS_VALS - 64);
                         }
+                        bits |= (st ? 1u : 0u) << i;
                     }
                     unsigned tot;
-                    const unsigned pos = group_excl_scan<WS_BS>(st ? 1u : 0u, red, tot);
-                    if (st && cnt_st + (int)pos <= WS_WIN)
-                        wst[cnt_st + pos] = (unsigned)k;
+                    unsigned pos = group_excl_scan<WS_BS>((unsigned)__popc(bits), red, tot);
+                    while (bits) { // this lane's starts, in order
+                        if (cnt_st + (int)pos <= WS_WIN)
+                            wst[cnt_st + pos] = (unsigned)(kl + __ffs(bits) - 1);
+                        ++pos;
+                        bits &= bits - 1;
+                    }
                     cnt_st += (int)uniform(tot);
                     __syncthreads(); // red is reused by the next scan
                 }
@@ -1362,7 +1454,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                     nact_s[buf ^ 1] = 0;
                     ntask_s[buf ^ 1] = 0;
                 }
-                for (int t = l >> 4; t < ntask; t += WS_BS / 16) {
+                for (int t = l / WS_GL; t < ntask; t += WS_BS / WS_GL) {
                     const unsigned tk = task[t];
                     const int te = (int)(tk & 511u), tc = (int)(tk >> 9);
                     const int k = cur[te] + tc * WS_CH;
@@ -1372,7 +1464,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                     double v[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        const int p = k + ll + 16 * u;
+                        const int p = k + ll + WS_GL * u;
                         c[u] = INT_MAX;
                         v[u] = 0.0;
                         if (p < ke) {
@@ -1386,12 +1478,12 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                         const bool in = c[u] < chi;
                         if (in)
                             add_c(c[u], v[u], a);
-                        took += __popcll((__ballot(in) >> lg0) & 0xffffull);
+                        took += __popcll((__ballot(in) >> lg0) & ((1ull << WS_GL) - 1ull));
                     }
                     if (took < WS_CH) { // the prefix ends here: offer (position, column) after it
 #pragma unroll
                         for (int u = 0; u < 4; ++u)
-                            if (ll + 16 * u == took)
+                            if (ll + WS_GL * u == took)
                                 atomicMin((unsigned long long *)&pc[te],
                                           ((u64)(unsigned)(k + took) << 32) | (unsigned)c[u]);
                     }
@@ -1419,58 +1511,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 }
                 has = again;
             }
-            // Output: columns first, then values.  The only loads of this phase (tile numbers
-            // of a two-level row, from ft) come before any store of a tile, and a flat row has
-            // none: a wait for a load also waits for every store issued before it.  A tile
-            // with at most 4 columns is written by one lane; denser tiles are listed (task[]
-            // is free here) and written by a wave each, one lane per column.
-            if (first) {
-                if (l == 0)
-                    nact_s[0] = 0;
-                __syncthreads();
-                auto put_cols = [&](auto tile_of) {
-                    int tid[3];
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const int q = l + i * WS_BS;
-                        tid[i] = q < nk ? tile_of(q) : 0;
-                    }
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const int q = l + i * WS_BS;
-                        u64 m = q < nk ? wm[q] : 0ull;
-                        const bool dense = __popcll(m) > 4;
-                        const u64 md = __ballot(dense);
-                        int d0 = 0;
-                        if (lane == 0 && md)
-                            d0 = atomicAdd(&nact_s[0], __popcll(md));
-                        d0 = __shfl(d0, 0, 64);
-                        if (dense) {
-                            task[d0 + mbcnt(md)] = (unsigned)q;
-                        } else if (m) {
-                            const int tile = mn + tid[i];
-                            long long at = cb + base + wo[q];
-                            while (m) {
-                                Ccol[at++] = (tile << 6) + __ffsll((unsigned long long)m) - 1;
-                                m &= m - 1;
-                            }
-                        }
-                    }
-                    __syncthreads();
-                    const int nd = uniform(nact_s[0]);
-                    for (int t = w; t < nd; t += WS_BS / 64) {
-                        const int q = (int)task[t];
-                        const u64 m = wm[q];
-                        const int tile = mn + tile_of(q);
-                        if ((m >> lane) & 1ull)
-                            Ccol[cb + base + wo[q] + __popcll(m & below(lane))] = (tile << 6) + lane;
-                    }
-                };
-                if (two)
-                    put_cols([&](int q) { return ft[k0 + q]; });
-                else
-                    put_cols([&](int q) { return k0 + q; });
-            }
+            // Output: the window's values (its columns were written by k_wide_cols).
             if (split) {
                 for (int q = l; q < cnt; q += WS_BS) {
                     const double v = vals[q];
@@ -1609,8 +1650,9 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
             hipLaunchKernelGGL(k_wide_order, dim3(1), dim3(64), 0, s, wide.ctr);
             hipLaunchKernelGGL(k_wide_fill, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, wide.items,
                                wide.items_cap, wide.ctr, err);
-            hipLaunchKernelGGL(k_wide_zero, dim3((unsigned)(count < 8192 ? count : 8192)), dim3(256), 0, s, rows,
-                               count_dev, op.Arp, crp, (double *)Cval);
+            hipLaunchKernelGGL(k_wide_cols, dim3((unsigned)(count < 8192 ? count : 8192)), dim3(WC_BS), 0, s, rows,
+                               count_dev, op, crp, Ccol, (double *)Cval, wide.sym_row_max, wide.sym_all_global,
+                               wide.long_rows);
             hipLaunchKernelGGL(k_numeric_wsplit, dim3((unsigned)wide.blocks), dim3(WS_BS), WS_LDS, s, wide.items, wide.ctr,
                                wide.ctr + 1, op, crp, Ccol, (double *)Cval, wide.ws, wide.ft_stride, wide.sym_row_max,
                                wide.sym_all_global, wide.long_rows, err);
