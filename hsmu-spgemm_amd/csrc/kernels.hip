@@ -146,28 +146,45 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(long long *bsum, int64_t n
         bsum[nb] = carry_s;
 }
 
+// Each thread scans SCAN_IPT consecutive elements; the block's tile passes through LDS
+// (padded by one element per SCAN_IPT, so a thread's run and a wave's coalesced row both
+// spread over the banks) so that global loads and stores stay coalesced.
+DEVI int scan_pad(int k) { return k + k / SCAN_IPT; }
+
 template <typename Tin, typename Tout>
 __global__ __launch_bounds__(SCAN_BS) void k_scan_apply(const Tin *__restrict__ in, int64_t n, const long long *__restrict__ bsum,
                                                          int64_t nb, Tout *__restrict__ out, long long *total_copy)
 {
     __shared__ long long red[SCAN_BS / 64];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_IPT;
+    __shared__ Tout tile[SCAN_TILE + SCAN_TILE / SCAN_IPT];
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_TILE;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; ++i) {
+        const int q = i * SCAN_BS + t;
+        tile[scan_pad(q)] = b0 + q < n ? (Tout)in[b0 + q] : (Tout)0;
+    }
+    __syncthreads();
     long long v[SCAN_IPT];
     long long loc = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_IPT; ++i) {
-        const int64_t k = base + i;
-        v[i] = k < n ? (long long)in[k] : 0;
+        v[i] = (long long)tile[scan_pad(t * SCAN_IPT + i)];
         loc += v[i];
     }
     long long total;
     long long pre = group_excl_scan<SCAN_BS>(loc, red, total) + bsum[blockIdx.x];
 #pragma unroll
     for (int i = 0; i < SCAN_IPT; ++i) {
-        const int64_t k = base + i;
-        if (k < n)
-            out[k] = (Tout)pre;
+        tile[scan_pad(t * SCAN_IPT + i)] = (Tout)pre;
         pre += v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; ++i) {
+        const int q = i * SCAN_BS + t;
+        if (b0 + q < n)
+            out[b0 + q] = tile[scan_pad(q)];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         out[n] = (Tout)bsum[nb];
@@ -203,7 +220,8 @@ hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *s
 //   2. k_bhead:  per element, validate the column and sortedness and flag tile heads
 //                (first element of a row, or tile column different from the left neighbour);
 //   3. exclusive scan of the flags: the tile index of every head;
-//   4. k_bfill:  per head, write its tile column and OR the bits of its run;
+//   4. k_bfill:  per head, write its tile column and OR the bits of its run (k_bcarry
+//                adds the parts of runs that cross a wave);
 //      k_btptr:  per row, tptr[j] = tile index at the row's first element.
 // Reference: compute_tile64_nums / compute_tile64_col / compute_tile64_mask_num,
 // src/compressed/compressed_mask64.cuh:61-342.
@@ -259,10 +277,12 @@ __global__ __launch_bounds__(256) void k_bhead(int64_t nnz, int64_t N, const int
 }
 
 // The bits of a run are OR-ed across the wave with shuffles (a segmented suffix OR: lane
-// i gathers [i, end of its run within the wave) in six doubling steps); a run that goes on
-// past the wave's last lane is finished serially by its head (one per wave at most).
+// i gathers [i, end of its run within the wave) in six doubling steps).  A run that goes
+// on past the wave's last lane is finished by k_bcarry: every wave whose first element is
+// not a head leaves the OR of its lanes up to its first head in part[wave].
 __global__ __launch_bounds__(256) void k_bfill(int64_t nnz, const int *__restrict__ col, const int *__restrict__ flag,
-                                               const int *__restrict__ tix, int *__restrict__ tcol, u64 *__restrict__ tmask)
+                                               const int *__restrict__ tix, int *__restrict__ tcol, u64 *__restrict__ tmask,
+                                               u64 *__restrict__ part)
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -280,15 +300,26 @@ __global__ __launch_bounds__(256) void k_bfill(int64_t nnz, const int *__restric
         if (lane + o < send)
             acc |= other;
     }
+    if (lane == 0 && !head)
+        part[k >> 6] = acc;
     if (!valid || !head)
         return;
-    if (send == 64) { // the run continues in the next wave
-        for (int64_t q = k + (64 - lane); q < nnz && !flag[q]; ++q)
-            acc |= 1ull << (col[q] & 63);
-    }
     const int t = tix[k];
     tcol[t] = c >> 6;
     tmask[t] = acc;
+}
+
+// Per wave of k_bfill whose first element continues a run: OR its part into that run's
+// tile (number tix - 1: the heads before it, less one).  Several waves may continue one
+// run (a tile with repeated columns), hence the atomic.
+__global__ __launch_bounds__(256) void k_bcarry(int64_t nnz, const int *__restrict__ flag, const int *__restrict__ tix,
+                                                const u64 *__restrict__ part, u64 *__restrict__ tmask)
+{
+    const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t k = w << 6;
+    if (k >= nnz || flag[k] != 0)
+        return;
+    atomicOr((unsigned long long *)&tmask[tix[k] - 1], (unsigned long long)part[w]);
 }
 
 __global__ __launch_bounds__(256) void k_btptr(int64_t K, int64_t nnz, const int *__restrict__ rp, const int *__restrict__ tix,
@@ -306,7 +337,7 @@ __global__ __launch_bounds__(256) void k_btptr(int64_t K, int64_t nnz, const int
         st->ntiles_b = tix[r];
 }
 
-size_t btiles_ws_bytes(int64_t nnz) { return (size_t)(2 * nnz + 2) * sizeof(int); }
+size_t btiles_ws_bytes(int64_t nnz) { return (size_t)(2 * nnz + 2) * sizeof(int) + (size_t)(nnz / 64 + 1) * sizeof(u64); }
 
 hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const int *col, int *ws, int *tptr,
                          int *tcol, u64 *tmask, void *scan_ws, DevStats *st, int num_cus, hipStream_t s)
@@ -314,6 +345,7 @@ hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const
     (void)num_cus;
     int *flag = ws;           // nnz + 1
     int *tix = ws + nnz + 1;  // nnz + 1
+    u64 *part = (u64 *)(ws + 2 * nnz + 2); // nnz / 64 + 1 (8-byte aligned: 2 nnz + 2 ints before it)
     const unsigned gk = (unsigned)((K + 256) / 256);
     const unsigned ge = (unsigned)((nnz + 255) / 256 > 0 ? (nnz + 255) / 256 : 1);
     hipError_t e = launch_fill_zero(flag, (size_t)(nnz + 1) * sizeof(int), s);
@@ -324,7 +356,8 @@ hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const
     e = scan_excl<int, int>(flag, nnz, tix, scan_ws, nullptr, s);
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_bfill, dim3(ge), dim3(256), 0, s, nnz, col, flag, tix, tcol, tmask);
+    hipLaunchKernelGGL(k_bfill, dim3(ge), dim3(256), 0, s, nnz, col, flag, tix, tcol, tmask, part);
+    hipLaunchKernelGGL(k_bcarry, dim3((unsigned)((nnz / 64 + 256) / 256)), dim3(256), 0, s, nnz, flag, tix, part, tmask);
     hipLaunchKernelGGL(k_btptr, dim3(gk), dim3(256), 0, s, K, nnz, rp, tix, tptr, st);
     return hipGetLastError();
 }
