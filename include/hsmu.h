@@ -113,7 +113,9 @@ typedef struct hsmu_opts {
                                  into one workgroup (one wave); 0 = default (8 KiB) */
     int32_t products_per_lane;/* numeric work per lane that sets a row's lane count; 0 = 32 */
     int32_t tiles_per_lane;   /* symbolic work per lane that sets a row's lane count; 0 = 4 */
-    int32_t timing;           /* 1: record per-stage hipEvents into hsmu_stats */
+    int32_t timing;           /* hipEvents into hsmu_stats (each costs the device a few us):
+                                 1 = per stage and per numeric bin (the default), 2 = per
+                                 numeric bin only, 0 = none (stage and bin times read 0) */
     int32_t b_lanes;          /* lanes that walk one row of B in the numeric phase (power of
                                  two, 1..64); 0 = a quarter of the mean row length of B */
     int32_t t_lanes;          /* lanes that walk one row of B's tiles in the tile OR (power
@@ -142,7 +144,7 @@ typedef struct hsmu_stats {
     double ms_form_cval;      /* numeric: accumulate, write columns and values */
     double ms_time2;          /* the reference's time2: all of the above but conversion */
     double ms_total;          /* conversion + time2 */
-    double ms_numeric_bin[HSMU_NBINS]; /* per numeric bin kernel time (timing = 1) */
+    double ms_numeric_bin[HSMU_NBINS]; /* per numeric bin kernel time (timing 1 or 2) */
     int64_t products;         /* intermediate products, sum over a_ij of nnz(B_j) */
     int64_t tile_products;    /* sum over a_ij of tiles(B_j) */
     int64_t nnz_c;
