@@ -1083,7 +1083,8 @@ __global__ void k_wide_order(unsigned *ctr)
 // pass 3: the items, at per-workgroup ranges of their class
 __global__ __launch_bounds__(256) void k_wide_fill(const int *__restrict__ rows, const int *__restrict__ count_p,
                                                    const int *__restrict__ Arp, const long long *__restrict__ crp,
-                                                   WideItem *__restrict__ items, long long cap, unsigned *ctr, int *err)
+                                                   WideItem *__restrict__ items, long long cap, unsigned *ctr,
+                                                   unsigned long long *sctr, long long scratch_n, int *err)
 {
     __shared__ unsigned lc[WS_CLASSES], lb[WS_CLASSES];
     if (threadIdx.x < WS_CLASSES)
@@ -1108,10 +1109,20 @@ __global__ __launch_bounds__(256) void k_wide_fill(const int *__restrict__ rows,
         atomicOr(err, ERR_INTERNAL);
         return;
     }
+    // a split row's range of the compensated scratch (sctr: nullptr for FP64)
+    long long sbase = 0;
+    if (sctr && ns > 1) {
+        const long long n = crp[row + 1] - crp[row];
+        sbase = (long long)atomicAdd(sctr, (unsigned long long)n);
+        if (sbase + n > scratch_n) { // cannot happen: scratch_n sums these rows' nnz
+            atomicOr(err, ERR_INTERNAL);
+            return;
+        }
+    }
     for (int q = 0; q < ns; ++q) {
         const int e0 = q * sz;
         const int e1 = e0 + sz < nA ? e0 + sz : nA;
-        items[base + q] = WideItem{row, e0, e1, ns};
+        items[base + q] = WideItem{row, e0, e1, ns, sbase};
     }
 }
 
@@ -1119,8 +1130,9 @@ constexpr int WC_BS = 256;    // lanes of a k_wide_cols workgroup
 constexpr int WC_DENSE = 1024; // dense tiles listed per pass (more are written by their lane)
 
 // Columns of the wide rows, from their kept directories, before k_numeric_wsplit adds the
-// values; and zeroed values for the rows split into several items (their items add into
-// C).  One workgroup per row.  Flat directory: a lane per tile; two-level: a lane per coarse
+// values; and, for FP64 (Cval given), zeroed values for the rows split into several items
+// (their items add into C; the compensated form's meet in a zeroed scratch instead).  One
+// workgroup per row.  Flat directory: a lane per tile; two-level: a lane per coarse
 // word, walking its tiles.  A tile with at most 4 columns is written by its lane; denser
 // tiles are listed in LDS and written by a wave each, one lane per column.
 __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ rows, const int *__restrict__ count_p,
@@ -1140,7 +1152,7 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
         const long long n = uniform64(crp[row + 1]) - cb;
         if (n <= 0)
             continue;
-        if (nA > wide_split(n))
+        if (Cval && nA > wide_split(n))
             for (long long q = l; q < n; q += WC_BS)
                 Cval[cb + q] = 0.0;
         const int mn = uniform(op.rmin[row]);
@@ -1215,6 +1227,25 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
 // in LDS when there are at most WS_COARSE of them.  Items of a split row add their window
 // sums into C (zeroed by k_wide_cols, which also writes every wide row's columns) with
 // atomics; the other rows store them.
+// (S, C) += (s, c) on a compensated pair in global memory, one 64-bit compare-and-swap
+// per attempt: TwoSum of S and s (as in PrecF32C::add), its error and c go into C.
+DEVI void merge_pair(u64 *w, float s, float c)
+{
+    u64 old = *w;
+    while (true) {
+        const float S = __uint_as_float((unsigned)old), Cc = __uint_as_float((unsigned)(old >> 32));
+        const float t = S + s;
+        const float bb = t - S;
+        const float err = (S - (t - bb)) + (s - bb);
+        const float c2 = Cc + (err + c);
+        const u64 nw = (u64)__float_as_uint(t) | ((u64)__float_as_uint(c2) << 32);
+        const u64 prev = atomicCAS((unsigned long long *)w, (unsigned long long)old, (unsigned long long)nw);
+        if (prev == old)
+            break;
+        old = prev;
+    }
+}
+
 constexpr int WS_COARSE = 256;
 constexpr int WS_BS = 768;   // lanes of a workgroup (two fit a CU: 6 waves per SIMD)
 constexpr int WS_WIN = 256;  // window starts listed at a time
@@ -1228,13 +1259,16 @@ constexpr int WS_LDS = (WS_VALS * 8 + WS_TILES * 8 + WS_SPLIT * 8 /* av */ + WS_
 constexpr u64 WS_NOPOS = 0xffffffffull << 32; // (position, column) with no position
 static_assert(2 + 2 * WS_CLASSES <= WIDE_CTRS, "wide-row counters");
 
+template <typename P>
 __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void k_numeric_wsplit(
     const WideItem *__restrict__ items, const unsigned *__restrict__ ctr_listed, unsigned *ctr_taken, Operands op,
-    const long long *__restrict__ crp, int *__restrict__ Ccol, double *__restrict__ Cval, int *ws,
+    const long long *__restrict__ crp, typename P::V *__restrict__ Cval, u64 *__restrict__ scratch, int *ws,
     long long ft_stride, long long sym_row_max, int sym_all_global, int long_rows, int *err)
 {
-    const double *__restrict__ Aval = (const double *)op.Aval;
-    const double *__restrict__ Bval = (const double *)op.Bval;
+    typedef typename P::V V;
+    constexpr bool F32C = std::is_same<P, PrecF32C>::value;
+    const V *__restrict__ Aval = (const V *)op.Aval;
+    const V *__restrict__ Bval = (const V *)op.Bval;
     extern __shared__ __align__(16) unsigned char lds[];
     double *vals = (double *)lds;
     u64 *wm = (u64 *)(vals + WS_VALS);
@@ -1266,6 +1300,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
             break;
         const int row = uniform(items[it].row), ie0 = uniform(items[it].e0);
         const int ne = uniform(items[it].e1) - ie0, nsplit = uniform(items[it].nsplit);
+        const long long sbase = uniform64(items[it].sbase);
         const int a0 = uniform(op.Arp[row]), nA = uniform(op.Arp[row + 1]) - a0;
         const long long cb = uniform64(crp[row]);
         const long long n = uniform64(crp[row + 1]) - cb;
@@ -1427,7 +1462,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                     act[buf * WS_SPLIT + ab + mbcnt(mh)] = e;
                 }
             };
-            auto add_c = [&](int c, double v, double a) {
+            auto add_c = [&](int c, V v, double a) {
                 const int tt = (c >> 6) - mn;
                 int r = tt - k0;
                 if (two) {
@@ -1436,7 +1471,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                     const unsigned o = clds ? co[q] : cog[q];
                     r = (int)(o + (unsigned)__popcll(m & below(tt & 63))) - k0;
                 }
-                atomicAdd(&vals[(unsigned)wo[r] + (unsigned)__popcll(wm[r] & below(c & 63))], a * v);
+                P::add(vals, (unsigned)wo[r] + (unsigned)__popcll(wm[r] & below(c & 63)), a, v);
             };
             // round 0: the entries with elements in this window; then those that go again
             bool has = l < ne && (int)(unsigned)pc[l] < chi;
@@ -1461,12 +1496,12 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                     const int ke = kend[te];
                     const double a = av[te];
                     int c[4];
-                    double v[4];
+                    V v[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int p = k + ll + WS_GL * u;
                         c[u] = INT_MAX;
-                        v[u] = 0.0;
+                        v[u] = 0;
                         if (p < ke) {
                             c[u] = op.Bcol[p];
                             v[u] = Bval[p];
@@ -1511,8 +1546,21 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 }
                 has = again;
             }
-            // Output: the window's values (its columns were written by k_wide_cols).
-            if (split) {
+            // Output: the window's values (its columns were written by k_wide_cols).  Split
+            // rows add theirs into C (FP64) or merge their (s, c) pairs into the scratch.
+            if constexpr (F32C) {
+                const float *fv = (const float *)vals;
+                if (split) {
+                    for (int q = l; q < cnt; q += WS_BS) {
+                        const float sv = fv[2 * q], cv = fv[2 * q + 1];
+                        if (sv != 0.0f || cv != 0.0f)
+                            merge_pair(&scratch[sbase + base + q], sv, cv);
+                    }
+                } else {
+                    for (int q = l; q < cnt; q += WS_BS)
+                        Cval[cb + base + q] = fv[2 * q] + fv[2 * q + 1];
+                }
+            } else if (split) {
                 for (int q = l; q < cnt; q += WS_BS) {
                     const double v = vals[q];
                     if (v != 0.0)
@@ -1528,6 +1576,28 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
         if (l == 0 && done != n)
             atomicOr(err, ERR_INTERNAL);
         block_sync_global(); // ft and the LDS state of the next item
+    }
+}
+
+// Compensated FP32, rows split into several items: C's values from the merged (s, c)
+// pairs.  A workgroup per row; the row's first item names it and its scratch range.
+__global__ __launch_bounds__(256) void k_wide_final(const WideItem *__restrict__ items,
+                                                    const unsigned *__restrict__ ctr_listed,
+                                                    const long long *__restrict__ crp, float *__restrict__ Cval,
+                                                    const u64 *__restrict__ scratch)
+{
+    const unsigned nitems = *ctr_listed;
+    for (unsigned idx = blockIdx.x; idx < nitems; idx += gridDim.x) {
+        const int e0 = uniform(items[idx].e0), ns = uniform(items[idx].nsplit);
+        if (e0 != 0 || ns <= 1)
+            continue;
+        const int row = uniform(items[idx].row);
+        const long long sb = uniform64(items[idx].sbase);
+        const long long cb = uniform64(crp[row]), n = uniform64(crp[row + 1]) - cb;
+        for (long long q = threadIdx.x; q < n; q += 256) {
+            const u64 w = scratch[sb + q];
+            Cval[cb + q] = __uint_as_float((unsigned)w) + __uint_as_float((unsigned)(w >> 32));
+        }
     }
 }
 
@@ -1641,21 +1711,29 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
 {
     if (bin == BIN_GLOBAL) {
         const int blocks = count < wide.blocks ? count : wide.blocks;
-        if (op.dir_mask && std::is_same<P, PrecF64>::value && wide.items) {
+        if (op.dir_mask && wide.items) { // split work items (FP64, or compensated with its scratch)
+            constexpr bool F32C = std::is_same<P, PrecF32C>::value;
             hipError_t e = launch_fill_zero(wide.ctr, WIDE_CTRS * sizeof(unsigned), s);
+            if (e == hipSuccess && F32C)
+                e = launch_fill_zero(wide.sctr, sizeof(unsigned long long), s);
+            if (e == hipSuccess && F32C && wide.scratch_n > 0)
+                e = launch_fill_zero(wide.scratch, (size_t)wide.scratch_n * sizeof(u64), s);
             if (e != hipSuccess)
                 return e;
             const unsigned gr = (unsigned)((count + 255) / 256);
             hipLaunchKernelGGL(k_wide_count, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, wide.ctr);
             hipLaunchKernelGGL(k_wide_order, dim3(1), dim3(64), 0, s, wide.ctr);
             hipLaunchKernelGGL(k_wide_fill, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, wide.items,
-                               wide.items_cap, wide.ctr, err);
+                               wide.items_cap, wide.ctr, F32C ? wide.sctr : nullptr, wide.scratch_n, err);
             hipLaunchKernelGGL(k_wide_cols, dim3((unsigned)(count < 8192 ? count : 8192)), dim3(WC_BS), 0, s, rows,
-                               count_dev, op, crp, Ccol, (double *)Cval, wide.sym_row_max, wide.sym_all_global,
-                               wide.long_rows);
-            hipLaunchKernelGGL(k_numeric_wsplit, dim3((unsigned)wide.blocks), dim3(WS_BS), WS_LDS, s, wide.items, wide.ctr,
-                               wide.ctr + 1, op, crp, Ccol, (double *)Cval, wide.ws, wide.ft_stride, wide.sym_row_max,
-                               wide.sym_all_global, wide.long_rows, err);
+                               count_dev, op, crp, Ccol, F32C ? nullptr : (double *)Cval, wide.sym_row_max,
+                               wide.sym_all_global, wide.long_rows);
+            hipLaunchKernelGGL(k_numeric_wsplit<P>, dim3((unsigned)wide.blocks), dim3(WS_BS), WS_LDS, s, wide.items,
+                               wide.ctr, wide.ctr + 1, op, crp, (typename P::V *)Cval, (u64 *)wide.scratch, wide.ws,
+                               wide.ft_stride, wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
+            if (F32C)
+                hipLaunchKernelGGL(k_wide_final, dim3(2048), dim3(256), 0, s, wide.items, wide.ctr, crp, (float *)Cval,
+                                   (const u64 *)wide.scratch);
         } else if (op.dir_mask)
             hipLaunchKernelGGL(k_numeric_wcompact<P>, dim3(blocks), dim3(1024), CW_LDS, s, rows, count_dev, op, crp,
                                Ccol, (typename P::V *)Cval, bshift > 6 ? 6 : bshift, wide.ws, wide.cur_stride,
@@ -1718,7 +1796,7 @@ hipError_t configure_kernels(int lds_max)
 #undef CFGP
 #undef CFGL
     CFG(k_numeric_window<PrecF64>) CFG(k_numeric_window<PrecF32C>)
-    CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>) CFG(k_numeric_wsplit)
+    CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>) CFG(k_numeric_wsplit<PrecF64>) CFG(k_numeric_wsplit<PrecF32C>)
 #undef CFG
     return e;
 }

@@ -815,11 +815,14 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             continue;
         // wide rows: a workspace per window workgroup (two 1024-lane workgroups fit a CU's
         // LDS) for the tiles of a two-level directory and, without split items, the cursors
-        // of its row's A entries.  FP64 rows with kept directories run as split work items
-        // (at most count + nnz(A) / WS_MIN_SPLIT of them) on a full persistent grid.
+        // of its row's A entries.  Rows with kept directories (FP64, and the compensated form
+        // when its scratch fits) run as split work items (at most count + nnz(A) /
+        // WS_MIN_SPLIT of them) on a full persistent grid.
         WideRows wide = {};
         if (b == BIN_GLOBAL) {
-            const bool split_items = keep_dirs && !fp32;
+            // the compensated form's split rows need an (s, c) pair per nonzero of theirs
+            const unsigned long long scratch_bytes = h2.wide_split_nnz * sizeof(unsigned long long);
+            const bool split_items = keep_dirs && (!fp32 || scratch_bytes <= c->total_mem / 4);
             wide.blocks = split_items || cnt > 2 * c->num_cus ? 2 * c->num_cus : cnt;
             wide.cur_stride = split_items ? 0 : ((long long)h2.num.max_na[b] + 63) & ~63LL;
             wide.ft_stride = keep_dirs ? (((long long)h2.num.max_span[b] + 63) & ~63LL) : 0;
@@ -831,8 +834,13 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
                 wide.items_cap = (long long)cnt + P.nnzA / WS_MIN_SPLIT + 1;
                 wide.items = ar.get<WideItem>((size_t)wide.items_cap);
                 wide.ctr = ar.get<unsigned>(WIDE_CTRS);
+                if (fp32) {
+                    wide.scratch_n = (long long)h2.wide_split_nnz;
+                    wide.scratch = ar.get<unsigned long long>((size_t)(wide.scratch_n > 0 ? wide.scratch_n : 1));
+                    wide.sctr = ar.get<unsigned long long>(1);
+                }
             }
-            if (!wide.ws || (split_items && (!wide.items || !wide.ctr)))
+            if (!wide.ws || (split_items && (!wide.items || !wide.ctr || (fp32 && (!wide.scratch || !wide.sctr)))))
                 return fail(HSMU_ERR_NOMEM, "wide-row workspace");
         }
         if (timing)

@@ -59,6 +59,7 @@ struct DevStats {
     int nlong; // rows of A set aside for the one-workgroup-per-row analysis
     int max_brow; // longest row of B (counted from LONG_TILES on)
     unsigned long long num_wide_hint; // rows whose numeric need may exceed LDS (symbolic binning)
+    unsigned long long wide_split_nnz; // nnz(C_i) summed over the wide rows split into several items
     BinStats sym;
     BinStats num;
     // work-queue counters of the persistent LDS kernels: [phase][bin][XCD]
@@ -268,6 +269,7 @@ hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *s
 // (nsplit > 1); their partial sums meet in C's values by atomic adds (C zeroed first).
 struct WideItem {
     int row, e0, e1, nsplit;
+    long long sbase; // compensated FP32, split rows: the row's first (s, c) pair in the scratch
 };
 constexpr int WIDE_CTRS = 66;
 constexpr int WS_VALS = 4096;        // value slots of one window
@@ -296,6 +298,12 @@ struct WideRows {
     long long items_cap;
     WideItem *items;
     unsigned *ctr;
+    // compensated FP32 split rows: one (s, c) float pair per nonzero of those rows (zeroed
+    // before the items merge their windows into it), and the counter that hands out each
+    // row's range
+    unsigned long long *scratch;
+    long long scratch_n;
+    unsigned long long *sctr;
 };
 size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks);
 

@@ -618,6 +618,14 @@ __global__ __launch_bounds__(1024) void k_bin(int numeric, int64_t M, const int 
     const u64 wides = __ballot(wide);
     if (lane == 0 && wides)
         atomicAdd(&st->num_wide_hint, (unsigned long long)__popcll(wides));
+    // nonzeros of the wide rows whose work is split into several items (their compensated
+    // FP32 partial sums meet in a scratch buffer of this size)
+    if (numeric) {
+        const unsigned long long sn =
+            wave_sum<unsigned long long>(b == BIN_GLOBAL && nA > wide_split(nc) ? (unsigned long long)nc : 0ull);
+        if (lane == 0 && sn)
+            atomicAdd(&st->wide_split_nnz, sn);
+    }
     int pos = 0;
     long long dloc = -1;
     u64 pending = __ballot(b >= 0);
@@ -936,7 +944,7 @@ __global__ void k_plan_check(DevStats *st, const DevStats *expect, int stage)
         same = same && st->dir_total == expect->dir_total && st->ntiles_b == expect->ntiles_b &&
                st->max_brow == expect->max_brow;
     else
-        same = same && st->nnz_c == expect->nnz_c;
+        same = same && st->nnz_c == expect->nnz_c && st->wide_split_nnz == expect->wide_split_nnz;
     if (!same)
         st->err |= ERR_PLAN;
     if (st->err) {

@@ -510,6 +510,21 @@ def test_fp32_compensated_duplicate_columns(ctx):
     check_fp32c(ctx, A2, Bd, "random with duplicated columns")
 
 
+@pytest.mark.parametrize("shape", [(4, 2000, 3000, 3000, 300000, 20, 80), (6, 1500, 2500, 5000, 60000, 100, 400),
+                                   (3, 3000, 3000, 6000, 1 << 21, 2, 2), "rmat12"])
+@pytest.mark.parametrize("opts", [dict(), dict(force_global=1)])
+def test_fp32_compensated_split_wide_rows(ctx, shape, opts):
+    """Compensated FP32 on wide rows split into several work items: each item's window
+    sums meet as (s, c) pairs in a scratch buffer (TwoSum by compare-and-swap) and are
+    rounded into C at the end."""
+    if shape == "rmat12":
+        A = hsmu.gen_rmat(12, 16, seed=2)
+        B = A
+    else:
+        A, B = split_rows_pair(5, *shape)
+    check_fp32c(ctx, A, B, f"split wide rows {shape}", **opts)
+
+
 @pytest.mark.parametrize("opts", [dict(force_global=1), dict(lds_row_max=512), dict(rebuild_dirs=1)])
 def test_fp32_compensated_fallbacks(ctx, opts):
     A = hsmu.gen_banded(3000, 20, 90, 700, seed=9)
