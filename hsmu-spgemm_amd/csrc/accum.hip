@@ -542,22 +542,30 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
                 if (blen[k] >= LONG_SEG)
                     lidx[atomicAdd(lcnt, 1)] = k;
         }
+        // One row per workgroup with its directory kept (a uniform condition): the slot
+        // offsets and totals are loaded, so neither the tile OR with its barrier nor the
+        // popcount scans run.
+        const bool known = G >= 64 && rd >= 0;
         if (rd < 0) {
             if (sparse)
                 or_tiles<G, OR_COARSE>(op, t0, tlen, nA, nullptr, 0, dm, nullptr, nullptr, mn, tshift);
             else
                 or_tiles<G, OR_FLAT>(op, t0, tlen, nA, nullptr, 0, dm, nullptr, nullptr, mn, tshift);
         }
-        __syncthreads();
         // first level: flat slot offsets (rebuilt) or coarse compact indices (rebuilt), and
         // the total (flat: nnz(C_i); two-level: non-empty tiles)
-        int ta, tb;
+        int ta = 0, tb = 0;
         unsigned tot;
-        unsigned pre = scan_popcounts<G>(dm, W, redu, tot, ta, tb);
-        if (rd < 0) {
-            for (int t = ta; t < tb; ++t) {
-                doff[t] = pre;
-                pre += (unsigned)__popcll(dm[t]);
+        if (known) {
+            tot = sparse ? (W > 0 ? doff[W - 1] + (unsigned)__popcll(dm[W - 1]) : 0u) : (unsigned)n;
+        } else {
+            __syncthreads();
+            unsigned pre = scan_popcounts<G>(dm, W, redu, tot, ta, tb);
+            if (rd < 0) {
+                for (int t = ta; t < tb; ++t) {
+                    doff[t] = pre;
+                    pre += (unsigned)__popcll(dm[t]);
+                }
             }
         }
         unsigned nnz = tot;
@@ -587,15 +595,20 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
             __syncthreads();
             if (sparse && rd < 0)
                 or_tiles<G, OR_FINE>(op, t0, tlen, nA, nullptr, 0, dm, doff, fm, mn, tshift);
-            __syncthreads();
-            unsigned tot2;
-            unsigned pre2 = scan_popcounts<G>(fm, sparse ? F : 0, redu, tot2, ta, tb);
-            if (sparse) {
-                nnz = tot2;
-                if (rd < 0) {
-                    for (int k = ta; k < tb; ++k) {
-                        foff[k] = pre2;
-                        pre2 += (unsigned)__popcll(fm[k]);
+            if (known) {
+                if (sparse)
+                    nnz = (unsigned)n;
+            } else {
+                __syncthreads();
+                unsigned tot2;
+                unsigned pre2 = scan_popcounts<G>(fm, sparse ? F : 0, redu, tot2, ta, tb);
+                if (sparse) {
+                    nnz = tot2;
+                    if (rd < 0) {
+                        for (int k = ta; k < tb; ++k) {
+                            foff[k] = pre2;
+                            pre2 += (unsigned)__popcll(fm[k]);
+                        }
                     }
                 }
             }
