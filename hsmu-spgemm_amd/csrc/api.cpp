@@ -822,7 +822,9 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         if (b == BIN_GLOBAL) {
             // the compensated form's split rows need an (s, c) pair per nonzero of theirs
             const unsigned long long scratch_bytes = h2.wide_split_nnz * sizeof(unsigned long long);
-            const bool split_items = keep_dirs && (!fp32 || scratch_bytes <= c->total_mem / 4);
+            const unsigned long long scratch_max =
+                opts.wide_scratch_mb > 0 ? (unsigned long long)opts.wide_scratch_mb << 20 : c->total_mem / 4;
+            const bool split_items = keep_dirs && (!fp32 || scratch_bytes <= scratch_max);
             wide.blocks = split_items || cnt > 2 * c->num_cus ? 2 * c->num_cus : cnt;
             wide.cur_stride = split_items ? 0 : ((long long)h2.num.max_na[b] + 63) & ~63LL;
             wide.ft_stride = keep_dirs ? (((long long)h2.num.max_span[b] + 63) & ~63LL) : 0;

@@ -512,11 +512,12 @@ def test_fp32_compensated_duplicate_columns(ctx):
 
 @pytest.mark.parametrize("shape", [(4, 2000, 3000, 3000, 300000, 20, 80), (6, 1500, 2500, 5000, 60000, 100, 400),
                                    (3, 3000, 3000, 6000, 1 << 21, 2, 2), "rmat12"])
-@pytest.mark.parametrize("opts", [dict(), dict(force_global=1)])
+@pytest.mark.parametrize("opts", [dict(), dict(force_global=1), dict(wide_scratch_mb=1)])
 def test_fp32_compensated_split_wide_rows(ctx, shape, opts):
     """Compensated FP32 on wide rows split into several work items: each item's window
     sums meet as (s, c) pairs in a scratch buffer (TwoSum by compare-and-swap) and are
-    rounded into C at the end."""
+    rounded into C at the end.  With the scratch capped at 1 MiB, rows whose scratch would
+    not fit take one workgroup per row (k_numeric_wcompact) instead."""
     if shape == "rmat12":
         A = hsmu.gen_rmat(12, 16, seed=2)
         B = A
