@@ -162,6 +162,73 @@ def test_batched_fp32_compensated(ctx):
         free_all(dA + dB)
 
 
+def wide_pair(seed, heavy, na, K, N, blen):
+    """A with `heavy` rows of `na` entries (then 50 rows of 0..20) over B (K x N, `blen`
+    entries per row): the heavy rows of C are wide and split into several work items."""
+    rng = np.random.default_rng(seed)
+    rows_b = [np.sort(rng.choice(N, size=blen, replace=False)).astype(np.int32) for _ in range(K)]
+    rp_b = np.zeros(K + 1, np.int32)
+    rp_b[1:] = np.cumsum([len(r) for r in rows_b])
+    B = hsmu.HostMatrix(K, N, rp_b, np.concatenate(rows_b), rng.uniform(-1, 1, int(rp_b[-1])))
+    rows_a = [np.sort(rng.choice(K, size=na if i < heavy else int(rng.integers(0, 21)), replace=False)).astype(np.int32)
+              for i in range(heavy + 50)]
+    rp_a = np.zeros(len(rows_a) + 1, np.int32)
+    rp_a[1:] = np.cumsum([len(r) for r in rows_a])
+    A = hsmu.HostMatrix(len(rows_a), K, rp_a, np.concatenate(rows_a), rng.uniform(-1, 1, int(rp_a[-1])))
+    return A, B
+
+
+def check_fp32c_views(ctx, cs, pairs):
+    for i, (A, B) in enumerate(pairs):
+        Cm = ctx.fetch(cs[i])
+        assert Cm.val.dtype == np.float32
+        ref = oracle.spgemm(A, B)
+        assert np.array_equal(np.asarray(Cm.rowptr, np.int64), ref.rowptr) and np.array_equal(Cm.col, ref.col)
+        bound = 2.0 ** -23 * np.abs(ref.val) + 1e-10 * ref.absval
+        assert np.all(np.abs(Cm.val.astype(np.float64) - ref.val) <= bound), f"instance {i}"
+
+
+def test_batched_and_plan_fp32c_split_wide_rows(ctx):
+    """Compensated FP32 with wide rows split into work items in several instances, batched
+    and as a plan replayed with new values: each split row's (s, c) scratch range is handed
+    out across the stacked instances, and the plan's capture sizes the scratch from the
+    count its recording run kept."""
+    R = as_fp32(hsmu.gen_rmat(11, 16, seed=2))
+    pairs = [tuple(as_fp32(m) for m in wide_pair(1, 3, 2000, 3000, 200000, 40)), (R, R),
+             tuple(as_fp32(m) for m in wide_pair(2, 2, 1500, 2500, 60000, 120))]
+    dA, dB = upload_pairs(ctx, pairs, val32=True)
+    o = hsmu.default_opts(precision=hsmu.HSMU_FP32_COMP)
+    note("fp32c batched split wide rows")
+    cs, _ = ctx.spgemm_batched_device(dA, dB, o)
+    try:
+        check_fp32c_views(ctx, cs, pairs)
+    finally:
+        ctx.free_batch(cs)
+    note("fp32c plan with split wide rows: record + capture")
+    plan = ctx.plan(dA, dB, o)
+    try:
+        check_fp32c_views(ctx, plan.results, pairs)
+        rng = np.random.default_rng(5)
+        for rep in range(2):
+            new_pairs = []
+            for (A, B), a, b in zip(pairs, dA, dB):
+                A2 = hsmu.HostMatrix(A.M, A.N, A.rowptr, A.col, rng.uniform(-1, 1, A.nnz).astype(np.float32).astype(np.float64))
+                set_values(ctx, a, A2.val.astype(np.float32))
+                if b is a:
+                    B2 = A2
+                else:
+                    B2 = hsmu.HostMatrix(B.M, B.N, B.rowptr, B.col,
+                                         rng.uniform(-1, 1, B.nnz).astype(np.float32).astype(np.float64))
+                    set_values(ctx, b, B2.val.astype(np.float32))
+                new_pairs.append((A2, B2))
+            note(f"fp32c plan replay {rep}")
+            plan.execute()
+            check_fp32c_views(ctx, plan.results, new_pairs)
+    finally:
+        plan.destroy()
+        free_all(dA + dB)
+
+
 def test_batched_all_empty(ctx):
     pairs = [(hsmu.gen_random(5 + i, 9, 0, i), hsmu.gen_random(9, 11, 3, i + 9)) for i in range(4)]
     dA, dB = upload_pairs(ctx, pairs)
