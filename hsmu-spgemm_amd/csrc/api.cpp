@@ -443,7 +443,7 @@ extern "C" int hsmu_btiles(hsmu_ctx *c, const hsmu_csr *B, hsmu_tiles *T)
         return fail(HSMU_ERR_NOMEM, "hsmu_btiles: workspace");
     }
     HIP_TRY(launch_fill_zero(dst, sizeof(DevStats), c->stream));
-    HIP_TRY(launch_btiles(K, B->N, B->nnz, (const int *)B->rowptr, B->col, tcnt, tptr, tcol, tmask, scan_ws, dst,
+    HIP_TRY(launch_btiles(K, B->N, B->nnz, (const int *)B->rowptr, B->col, tcnt, tptr, tcol, tmask, nullptr, scan_ws, dst,
                           c->num_cus, c->stream));
     HIP_TRY(hipMemcpyAsync(c->host_stats, dst, sizeof(DevStats), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -627,17 +627,19 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     int *rnnz = ar.get<int>((size_t)M);
     int *binrows = ar.get<int>((size_t)NBINS * (size_t)M);
     long long *rdir = ar.get<long long>((size_t)M);
-    if (!tcnt || !tptr || !tcol || !tmask || !scan_ws || !rprod || !rtp || !rmin || !rmax || !rnnz || !binrows || !rdir)
+    BRow *brow = ar.get<BRow>((size_t)(K > 0 ? K : 1));
+    if (!tcnt || !tptr || !tcol || !tmask || !scan_ws || !rprod || !rtp || !rmin || !rmax || !rnnz || !binrows || !rdir ||
+        !brow)
         return fail(HSMU_ERR_NOMEM, "workspace allocation");
 
     // ---- 1. B -> tiles ----
-    HIP_TRY(launch_btiles(K, N, P.nnzB, P.Brp, P.Bcol, tcnt, tptr, tcol, tmask, scan_ws, dst, c->num_cus, s));
+    HIP_TRY(launch_btiles(K, N, P.nnzB, P.Brp, P.Bcol, tcnt, tptr, tcol, tmask, brow, scan_ws, dst, c->num_cus, s));
     HIP_TRY(mark(c->ev[EV_CONV]));
 
     // ---- 2. row analysis + symbolic binning ----
     const int avg_a = (int)(P.nnzA / (M > 0 ? M : 1));
     // the spare bin region (bin 9) holds the list of long rows until binning runs
-    HIP_TRY(launch_row_info(M, K, P.nnzA, P.Arp, P.Acol, P.Brp, tptr, tcol, rprod, rtp, rmin, rmax, rnnz,
+    HIP_TRY(launch_row_info(M, K, P.nnzA, P.Arp, P.Acol, brow, rprod, rtp, rmin, rmax, rnnz,
                             binrows + (size_t)(NBINS - 1) * M, dst, avg_a, c->num_cus, s));
     BinParams bp;
     const int lds_dyn = c->lds_max - LDS_STATIC_RESERVE;

@@ -205,16 +205,21 @@ hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 size_t scan_ws_bytes(int64_t n);
 // ws must hold btiles_ws_bytes(nnz) bytes, scan_ws scan_ws_bytes(nnz + 1).
 size_t btiles_ws_bytes(int64_t nnz);
+// One row of B as the row analysis needs it: entries, tiles, first and last tile column
+// (last = -1 for an empty row).
+struct alignas(16) BRow {
+    int len, ntiles, first, last;
+};
+// brow (K entries, may be null): each row's BRow, written beside the tile row pointers.
 hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const int *col,
-                         int *ws, int *tptr, int *tcol, unsigned long long *tmask, void *scan_ws,
-                         DevStats *st, int num_cus, hipStream_t s);
+                         int *ws, int *tptr, int *tcol, unsigned long long *tmask, BRow *brow,
+                         void *scan_ws, DevStats *st, int num_cus, hipStream_t s);
 
-// Per row of A: products, tile-products, tile span.  longrows (M ints) is scratch for the
-// list of long rows analysed one workgroup per row.
+// Per row of A: products, tile-products, tile span (from the BRow of each B_j it names).
+// longrows (M ints) is scratch for the list of long rows analysed one workgroup per row.
 hipError_t launch_row_info(int64_t M, int64_t K, int64_t nnzA, const int *Arp, const int *Acol,
-                           const int *Brp, const int *tptr, const int *tcol, long long *rprod,
-                           int *rtp, int *rmin, int *rmax, int *rnnz, int *longrows, DevStats *st,
-                           int avg_a, int num_cus, hipStream_t s);
+                           const BRow *brow, long long *rprod, int *rtp, int *rmin, int *rmax,
+                           int *rnnz, int *longrows, DevStats *st, int avg_a, int num_cus, hipStream_t s);
 
 // Bin rows for the symbolic (numeric = 0) or numeric (numeric = 1) phase.  The symbolic
 // pass also reserves each LDS-bin row's directory range: rdir[i] = offset (span entries),

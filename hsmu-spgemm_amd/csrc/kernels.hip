@@ -322,25 +322,37 @@ __global__ __launch_bounds__(256) void k_bcarry(int64_t nnz, const int *__restri
     atomicOr((unsigned long long *)&tmask[tix[k] - 1], (unsigned long long)part[w]);
 }
 
-__global__ __launch_bounds__(256) void k_btptr(int64_t K, int64_t nnz, const int *__restrict__ rp, const int *__restrict__ tix,
-                                               int *__restrict__ tptr, DevStats *st)
+__global__ __launch_bounds__(256) void k_btptr(int64_t K, int64_t nnz, const int *__restrict__ rp,
+                                               const int *__restrict__ col, const int *__restrict__ tix,
+                                               int *__restrict__ tptr, BRow *__restrict__ brow, DevStats *st)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j > K)
         return;
     // a row pointer outside [0, nnz] has already raised ERR_ROWPTR in k_bmark; keep
-    // the read in bounds until the host sees the error
-    int r = rp[j];
-    r = r < 0 ? 0 : (r > nnz ? (int)nnz : r);
+    // the reads in bounds until the host sees the error
+    auto clamp = [&](int r) { return r < 0 ? 0 : (r > nnz ? (int)nnz : r); };
+    const int r = clamp(rp[j]);
     tptr[j] = tix[r];
-    if (j == K)
+    if (j == K) {
         st->ntiles_b = tix[r];
+    } else if (brow) {
+        const int r1 = clamp(rp[j + 1]);
+        BRow q = {0, 0, 0, -1};
+        if (r1 > r) {
+            q.len = r1 - r;
+            q.ntiles = tix[r1] - tix[r];
+            q.first = col[r] >> 6; // rows are sorted: their first and last tiles
+            q.last = col[r1 - 1] >> 6;
+        }
+        brow[j] = q;
+    }
 }
 
 size_t btiles_ws_bytes(int64_t nnz) { return (size_t)(2 * nnz + 2) * sizeof(int) + (size_t)(nnz / 64 + 1) * sizeof(u64); }
 
 hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const int *col, int *ws, int *tptr,
-                         int *tcol, u64 *tmask, void *scan_ws, DevStats *st, int num_cus, hipStream_t s)
+                         int *tcol, u64 *tmask, BRow *brow, void *scan_ws, DevStats *st, int num_cus, hipStream_t s)
 {
     (void)num_cus;
     int *flag = ws;           // nnz + 1
@@ -358,7 +370,7 @@ hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const
         return e;
     hipLaunchKernelGGL(k_bfill, dim3(ge), dim3(256), 0, s, nnz, col, flag, tix, tcol, tmask, part);
     hipLaunchKernelGGL(k_bcarry, dim3((unsigned)((nnz / 64 + 256) / 256)), dim3(256), 0, s, nnz, flag, tix, part, tmask);
-    hipLaunchKernelGGL(k_btptr, dim3(gk), dim3(256), 0, s, K, nnz, rp, tix, tptr, st);
+    hipLaunchKernelGGL(k_btptr, dim3(gk), dim3(256), 0, s, K, nnz, rp, col, tix, tptr, brow, st);
     return hipGetLastError();
 }
 
@@ -374,8 +386,7 @@ constexpr int LONG_A = 2048; // rows of A longer than this are analysed by a who
 
 template <int G>
 __global__ __launch_bounds__(256) void k_row_info(int64_t M, int64_t K, int64_t nnzA, const int *__restrict__ Arp,
-                                                  const int *__restrict__ Acol, const int *__restrict__ Brp,
-                                                  const int *__restrict__ tptr, const int *__restrict__ tcol,
+                                                  const int *__restrict__ Acol, const BRow *__restrict__ brow,
                                                   long long *__restrict__ rprod, int *__restrict__ rtp,
                                                   int *__restrict__ rmin, int *__restrict__ rmax, int *__restrict__ rnnz,
                                                   int *__restrict__ longrows, DevStats *st)
@@ -402,14 +413,12 @@ __global__ __launch_bounds__(256) void k_row_info(int64_t M, int64_t K, int64_t 
                 bad |= ERR_ACOL;
                 continue;
             }
-            const int b0 = Brp[j], b1 = Brp[j + 1];
-            const int t0 = tptr[j], t1 = tptr[j + 1];
-            if (b1 > b0 && t1 > t0) {
-                p += b1 - b0;
-                tp += t1 - t0;
-                const int f = tcol[t0], z = tcol[t1 - 1];
-                mn = f < mn ? f : mn;
-                mx = z > mx ? z : mx;
+            const BRow q = brow[j];
+            if (q.len > 0) {
+                p += q.len;
+                tp += q.ntiles;
+                mn = q.first < mn ? q.first : mn;
+                mx = q.last > mx ? q.last : mx;
             }
         }
     }
@@ -433,8 +442,7 @@ __global__ __launch_bounds__(256) void k_row_info(int64_t M, int64_t K, int64_t 
 // row, grid-stride over the device-side list.
 __global__ __launch_bounds__(1024) void k_row_info_long(int64_t K, const int *__restrict__ longrows,
                                                         const int *__restrict__ Arp, const int *__restrict__ Acol,
-                                                        const int *__restrict__ Brp, const int *__restrict__ tptr,
-                                                        const int *__restrict__ tcol, long long *__restrict__ rprod,
+                                                        const BRow *__restrict__ brow, long long *__restrict__ rprod,
                                                         int *__restrict__ rtp, int *__restrict__ rmin,
                                                         int *__restrict__ rmax, int *__restrict__ rnnz, DevStats *st)
 {
@@ -453,14 +461,12 @@ __global__ __launch_bounds__(1024) void k_row_info_long(int64_t K, const int *__
                 bad |= ERR_ACOL;
                 continue;
             }
-            const int b0 = Brp[j], b1 = Brp[j + 1];
-            const int t0 = tptr[j], t1 = tptr[j + 1];
-            if (b1 > b0 && t1 > t0) {
-                p += b1 - b0;
-                tp += t1 - t0;
-                const int f = tcol[t0], z = tcol[t1 - 1];
-                mn = f < mn ? f : mn;
-                mx = z > mx ? z : mx;
+            const BRow q = brow[j];
+            if (q.len > 0) {
+                p += q.len;
+                tp += q.ntiles;
+                mn = q.first < mn ? q.first : mn;
+                mx = q.last > mx ? q.last : mx;
             }
         }
         p = group_sum<64>(p, (long long *)nullptr);
@@ -499,16 +505,16 @@ __global__ __launch_bounds__(1024) void k_row_info_long(int64_t K, const int *__
     }
 }
 
-hipError_t launch_row_info(int64_t M, int64_t K, int64_t nnzA, const int *Arp, const int *Acol, const int *Brp,
-                           const int *tptr, const int *tcol, long long *rprod, int *rtp, int *rmin, int *rmax, int *rnnz,
-                           int *longrows, DevStats *st, int avg_a, int num_cus, hipStream_t s)
+hipError_t launch_row_info(int64_t M, int64_t K, int64_t nnzA, const int *Arp, const int *Acol, const BRow *brow,
+                           long long *rprod, int *rtp, int *rmin, int *rmax, int *rnnz, int *longrows, DevStats *st,
+                           int avg_a, int num_cus, hipStream_t s)
 {
 #define ROW_INFO(G)                                                                                                    \
     {                                                                                                                  \
         constexpr int RPB = 256 / G;                                                                                   \
         const int64_t blocks = (M + RPB - 1) / RPB;                                                                    \
         hipLaunchKernelGGL((k_row_info<G>), dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(256), 0, s, M, K, nnzA,    \
-                           Arp, Acol, Brp, tptr, tcol, rprod, rtp, rmin, rmax, rnnz, longrows, st);                    \
+                           Arp, Acol, brow, rprod, rtp, rmin, rmax, rnnz, longrows, st);                               \
     }
     if (avg_a <= 12)
         ROW_INFO(8)
@@ -519,8 +525,8 @@ hipError_t launch_row_info(int64_t M, int64_t K, int64_t nnzA, const int *Arp, c
     else
         ROW_INFO(64)
 #undef ROW_INFO
-    hipLaunchKernelGGL(k_row_info_long, dim3((unsigned)num_cus), dim3(1024), 0, s, K, longrows, Arp, Acol, Brp, tptr,
-                       tcol, rprod, rtp, rmin, rmax, rnnz, st);
+    hipLaunchKernelGGL(k_row_info_long, dim3((unsigned)num_cus), dim3(1024), 0, s, K, longrows, Arp, Acol, brow, rprod,
+                       rtp, rmin, rmax, rnnz, st);
     return hipGetLastError();
 }
 
