@@ -657,7 +657,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     bp.force_global = opts.force_global;
     bp.lds_cu = lds_dyn;
     bp.min_waves = opts.min_waves > 0 ? opts.min_waves : 8;
-    HIP_TRY(launch_bin(0, M, P.Arp, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, s));
+    HIP_TRY(launch_bin(0, M, P.Arp, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, c->num_cus, s));
     HIP_TRY(mark(c->ev[EV_ANALYSIS]));
     DevStats h1;
     rc = checkpoint(c, mode, dst, 0, h1);
@@ -774,7 +774,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         return fail(HSMU_ERR_NOMEM, "C row pointers");
     HIP_TRY(launch_scan_i32_i64(rnnz, M, crp, scan_ws, &dst->nnz_c, s));
     bp.per_lane = opts.products_per_lane > 0 ? opts.products_per_lane : 32;
-    HIP_TRY(launch_bin(1, M, P.Arp, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, s));
+    HIP_TRY(launch_bin(1, M, P.Arp, rprod, rtp, rmin, rmax, rnnz, bp, binrows, rdir, dst, c->num_cus, s));
     HIP_TRY(mark(c->ev[EV_SYMBOLIC]));
     DevStats h2;
     rc = checkpoint(c, mode, dst, 1, h2);

@@ -226,7 +226,8 @@ hipError_t launch_row_info(int64_t M, int64_t K, int64_t nnzA, const int *Arp, c
 // -1 for rows not in an LDS bin; st->dir_total = entries reserved.
 hipError_t launch_bin(int numeric, int64_t M, const int *Arp, const long long *rprod,
                       const int *rtp, const int *rmin, const int *rmax, const int *rnnz,
-                      BinParams bp, int *binrows, long long *rdir, DevStats *st, hipStream_t s);
+                      BinParams bp, int *binrows, long long *rdir, DevStats *st, int num_cus,
+                      hipStream_t s);
 
 struct Operands {
     const int *Arp, *Acol;

@@ -565,10 +565,12 @@ DEVI int choose_bin(long long work, long long need, const BinParams &bp)
 
 // One thread per row; appends are aggregated per workgroup in LDS, then one global
 // atomic per bin per workgroup reserves the range.  Also sums each bin's work, nnz(A)
-// and (numeric) nnz(C) for the per-kernel algorithmic byte counts.  Workgroups of BIN_BS
-// lanes, so that 62K rows already occupy every CU (1024-lane ones left three quarters idle).
-constexpr int BIN_BS = 256;
-__global__ __launch_bounds__(BIN_BS) void k_bin(int numeric, int64_t M, const int *__restrict__ Arp,
+// and (numeric) nnz(C) for the per-kernel algorithmic byte counts.  Workgroups of BS
+// lanes: 256 while that makes at most four per CU (62K rows then occupy every CU, where
+// 1024-lane ones left three quarters idle), 1024 beyond (millions of rows would otherwise
+// queue four times as many atomics on the same bin counters).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_bin(int numeric, int64_t M, const int *__restrict__ Arp,
                                               const long long *__restrict__ rprod, const int *__restrict__ rtp,
                                               const int *__restrict__ rmin, const int *__restrict__ rmax,
                                               const int *__restrict__ rnnz, BinParams bp, int *__restrict__ binrows,
@@ -590,7 +592,7 @@ __global__ __launch_bounds__(BIN_BS) void k_bin(int numeric, int64_t M, const in
     if (threadIdx.x == 0)
         s_dir = 0;
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * BIN_BS + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
     const bool with_long = st->max_brow >= LONG_TILES;
     int b = -1, S = 0, nA = 0, nc = 0, sp = 0;
     bool wide = false;
@@ -702,13 +704,17 @@ __global__ __launch_bounds__(BIN_BS) void k_bin(int numeric, int64_t M, const in
 
 hipError_t launch_bin(int numeric, int64_t M, const int *Arp, const long long *rprod, const int *rtp, const int *rmin,
                       const int *rmax, const int *rnnz, BinParams bp, int *binrows, long long *rdir, DevStats *st,
-                      hipStream_t s)
+                      int num_cus, hipStream_t s)
 {
-    const int64_t blocks = (M + BIN_BS - 1) / BIN_BS;
-    if (blocks == 0)
+    if (M <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_bin, dim3((unsigned)blocks), dim3(BIN_BS), 0, s, numeric, M, Arp, rprod, rtp, rmin, rmax, rnnz,
-                       bp, binrows, rdir, st);
+    const int64_t small = (M + 255) / 256;
+    if (small <= 4LL * num_cus)
+        hipLaunchKernelGGL(k_bin<256>, dim3((unsigned)small), dim3(256), 0, s, numeric, M, Arp, rprod, rtp, rmin, rmax,
+                           rnnz, bp, binrows, rdir, st);
+    else
+        hipLaunchKernelGGL(k_bin<1024>, dim3((unsigned)((M + 1023) / 1024)), dim3(1024), 0, s, numeric, M, Arp, rprod,
+                           rtp, rmin, rmax, rnnz, bp, binrows, rdir, st);
     return hipGetLastError();
 }
 
