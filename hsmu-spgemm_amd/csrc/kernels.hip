@@ -216,7 +216,8 @@ hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *s
 //
 // Element-parallel, so a row of any length costs the same per element (R-MAT's hub rows
 // made a row-per-group conversion wait on its longest row):
-//   1. k_bmark:  per row, validate the row pointers and flag the row's first element;
+//   1. k_bmark:  per row, validate the row pointers and mark the row's first element in a
+//                bitmap (only the bitmap is cleared beforehand, not the flags);
 //   2. k_bhead:  per element, validate the column and sortedness and flag tile heads
 //                (first element of a row, or tile column different from the left neighbour);
 //   3. exclusive scan of the flags: the tile index of every head;
@@ -227,7 +228,7 @@ hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *s
 // src/compressed/compressed_mask64.cuh:61-342.
 
 __global__ __launch_bounds__(256) void k_bmark(int64_t K, int64_t nnz, const int *__restrict__ rp,
-                                               int *__restrict__ flag, DevStats *st)
+                                               unsigned *__restrict__ starts, DevStats *st)
 {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     int len = 0;
@@ -238,7 +239,7 @@ __global__ __launch_bounds__(256) void k_bmark(int64_t K, int64_t nnz, const int
         } else {
             len = e - s;
             if (e > s)
-                flag[s] = 2;
+                atomicOr(&starts[s >> 5], 1u << (s & 31));
         }
     }
     // longest row, counted only from LONG_TILES on (all the kernels need to know): no
@@ -250,8 +251,8 @@ __global__ __launch_bounds__(256) void k_bmark(int64_t K, int64_t nnz, const int
     }
 }
 
-__global__ __launch_bounds__(256) void k_bhead(int64_t nnz, int64_t N, const int *__restrict__ col, int *__restrict__ flag,
-                                               DevStats *st)
+__global__ __launch_bounds__(256) void k_bhead(int64_t nnz, int64_t N, const int *__restrict__ col,
+                                               const unsigned *__restrict__ starts, int *__restrict__ flag, DevStats *st)
 {
     const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
     int bad = 0;
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(256) void k_bhead(int64_t nnz, int64_t N, const int
         const int c = col[k];
         if ((unsigned)c >= (unsigned long long)N)
             bad |= ERR_BCOL;
-        const bool start = flag[k] == 2;
+        const bool start = (starts[k >> 5] >> (k & 31)) & 1u;
         int head = 1;
         if (!start && k > 0) {
             const int pc = col[k - 1];
@@ -349,7 +350,11 @@ __global__ __launch_bounds__(256) void k_btptr(int64_t K, int64_t nnz, const int
     }
 }
 
-size_t btiles_ws_bytes(int64_t nnz) { return (size_t)(2 * nnz + 2) * sizeof(int) + (size_t)(nnz / 64 + 1) * sizeof(u64); }
+size_t btiles_ws_bytes(int64_t nnz)
+{
+    return (size_t)(2 * nnz + 2) * sizeof(int) + (size_t)(nnz / 64 + 1) * sizeof(u64) +
+           (size_t)(nnz / 32 + 1) * sizeof(unsigned);
+}
 
 hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const int *col, int *ws, int *tptr,
                          int *tcol, u64 *tmask, BRow *brow, void *scan_ws, DevStats *st, int num_cus, hipStream_t s)
@@ -358,13 +363,14 @@ hipError_t launch_btiles(int64_t K, int64_t N, int64_t nnz, const int *rp, const
     int *flag = ws;           // nnz + 1
     int *tix = ws + nnz + 1;  // nnz + 1
     u64 *part = (u64 *)(ws + 2 * nnz + 2); // nnz / 64 + 1 (8-byte aligned: 2 nnz + 2 ints before it)
+    unsigned *starts = (unsigned *)(part + nnz / 64 + 1); // nnz / 32 + 1 bits of row starts
     const unsigned gk = (unsigned)((K + 256) / 256);
     const unsigned ge = (unsigned)((nnz + 255) / 256 > 0 ? (nnz + 255) / 256 : 1);
-    hipError_t e = launch_fill_zero(flag, (size_t)(nnz + 1) * sizeof(int), s);
+    hipError_t e = launch_fill_zero(starts, (size_t)(nnz / 32 + 1) * sizeof(unsigned), s);
     if (e != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_bmark, dim3(gk), dim3(256), 0, s, K, nnz, rp, flag, st);
-    hipLaunchKernelGGL(k_bhead, dim3(ge), dim3(256), 0, s, nnz, N, col, flag, st);
+    hipLaunchKernelGGL(k_bmark, dim3(gk), dim3(256), 0, s, K, nnz, rp, starts, st);
+    hipLaunchKernelGGL(k_bhead, dim3(ge), dim3(256), 0, s, nnz, N, col, starts, flag, st);
     e = scan_excl<int, int>(flag, nnz, tix, scan_ws, nullptr, s);
     if (e != hipSuccess)
         return e;
