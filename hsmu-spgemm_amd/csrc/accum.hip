@@ -1139,15 +1139,19 @@ __global__ __launch_bounds__(256) void k_wide_fill(const int *__restrict__ rows,
     }
 }
 
-constexpr int WC_BS = 256;    // lanes of a k_wide_cols workgroup
+constexpr int WC_BS = 256;     // lanes of a k_wide_cols workgroup
 constexpr int WC_DENSE = 1024; // dense tiles listed per pass (more are written by their lane)
+constexpr int WC_STAGE = 4096; // columns of a flat chunk staged in LDS for coalesced stores
 
 // Columns of the wide rows, from their kept directories, before k_numeric_wsplit adds the
 // values; and, for FP64 (Cval given), zeroed values for the rows split into several items
 // (their items add into C; the compensated form's meet in a zeroed scratch instead).  One
-// workgroup per row.  Flat directory: a lane per tile; two-level: a lane per coarse
-// word, walking its tiles.  A tile with at most 4 columns is written by its lane; denser
-// tiles are listed in LDS and written by a wave each, one lane per column.
+// workgroup per row.  Flat directory: WC_BS consecutive tiles' columns are one contiguous
+// range of C, so a scan of the tiles' popcounts places each lane's columns in an LDS stage
+// that the workgroup then stores coalesced.  Chunks of more than WC_STAGE columns, and
+// two-level rows, take the direct path: a lane per tile (flat) or per coarse word,
+// walking its tiles (two-level); a tile with at most 4 columns is written by its lane,
+// denser tiles are listed in LDS and written by a wave each, one lane per column.
 __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ rows, const int *__restrict__ count_p,
                                                      Operands op, const long long *__restrict__ crp,
                                                      int *__restrict__ Ccol, double *__restrict__ Cval,
@@ -1155,6 +1159,8 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
 {
     __shared__ unsigned dk[WC_DENSE]; // dense tiles: fine entry
     __shared__ int dtile[WC_DENSE];   // and tile (absolute)
+    __shared__ int stage[WC_STAGE];
+    __shared__ unsigned red[WC_BS / 64];
     __shared__ int nd_s;
     const int l = threadIdx.x, lane = l & 63, w = l >> 6;
     const int nrows = *count_p;
@@ -1195,10 +1201,30 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
         };
         const int U = two ? W : S; // units: coarse words, or tiles
         for (int u0 = 0; u0 < U; u0 += WC_BS) {
+            const int u = u0 + l;
+            if (!two) { // staged: [foff[u0], foff[u0] + tot) holds this chunk's columns
+                const u64 m = u < S ? fm[u] : 0ull;
+                unsigned tot;
+                const unsigned pos = group_excl_scan<WC_BS>((unsigned)__popcll(m), red, tot);
+                if (uniform(tot) <= (unsigned)WC_STAGE) {
+                    const unsigned base = uniform(foff[u0]);
+                    const int c0 = (mn + u) << 6;
+                    u64 mm = m;
+                    unsigned q = pos;
+                    while (mm) {
+                        stage[q++] = c0 + __ffsll((unsigned long long)mm) - 1;
+                        mm &= mm - 1;
+                    }
+                    __syncthreads();
+                    for (unsigned i = l; i < tot; i += WC_BS)
+                        Ccol[cb + base + i] = stage[i];
+                    __syncthreads(); // the stage of the next chunk
+                    continue;
+                }
+            }
             if (l == 0)
                 nd_s = 0;
             __syncthreads();
-            const int u = u0 + l;
             if (u < U) {
                 if (two) {
                     u64 cw = cmg[u];
