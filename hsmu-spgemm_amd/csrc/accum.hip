@@ -631,8 +631,9 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
         }
         __syncthreads();
         if (row >= 0 && ok) {
-            if constexpr (G >= 64) {
-                constexpr int NW = G / 64;
+            if (G >= 64 && (dense || (unsigned)n >= 16u * (unsigned)F)) {
+                // tiles holding 16 columns or more on average: a wave per tile, a lane per column
+                constexpr int NW = G < 64 ? 1 : G / 64;
                 const int w = l >> 6, lane = l & 63;
                 for (int k = w; k < F; k += NW) {
                     const u64 m = fm[k];
@@ -643,7 +644,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
                         Cval[cb + pos] = P::result(vals, dense ? (t << 6) + lane : pos);
                     }
                 }
-            } else {
+            } else { // a lane per tile, then the values in slot order
                 for (int k = l; k < F; k += G) {
                     u64 m = fm[k];
                     unsigned pos = foff[k];
