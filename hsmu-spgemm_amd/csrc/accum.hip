@@ -140,20 +140,55 @@ DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, c
     }
 }
 
-// Each lane scans a contiguous chunk of the n directory entries: the group's exclusive
-// prefix of the popcounts at the chunk start, the group total in tot.  Every thread of the
-// workgroup must call it (G > 64 synchronises the workgroup).
+// Each lane scans a contiguous chunk [ta, tb) of the n directory entries; groups of 256
+// lanes or more visit it from s = ta + l mod chunk, wrapping round to ta: with every lane
+// starting at its chunk's first entry, a chunk of 16 entries would put all 64 lanes of a
+// wave on one LDS bank pair (smaller groups have short chunks and start at ta).  pre: the group's exclusive prefix of the popcounts at ta; head: the popcounts of
+// [ta, s); tot: the group total.  Every thread of the workgroup must call it (G > 64
+// synchronises the workgroup).
+struct ChunkScan {
+    int ta, tb, s;
+    unsigned pre, head, tot;
+};
 template <int G>
-DEVI unsigned scan_popcounts(const u64 *m, int n, unsigned *redu, unsigned &tot, int &ta, int &tb)
+DEVI ChunkScan scan_popcounts(const u64 *m, int n, unsigned *redu)
 {
+    ChunkScan c;
     const int l = threadIdx.x & (G - 1);
     const int chunk = (n + G - 1) / G;
-    ta = l * chunk;
-    tb = ta + chunk < n ? ta + chunk : n;
-    unsigned loc = 0;
-    for (int t = ta; t < tb; ++t)
-        loc += (unsigned)__popcll(m[t]);
-    return group_excl_scan<G>(loc, redu, tot);
+    c.ta = l * chunk;
+    c.tb = c.ta + chunk < n ? c.ta + chunk : n;
+    c.s = G >= 256 && chunk > 1 ? c.ta + l % chunk : c.ta;
+    if (c.s > c.tb)
+        c.s = c.tb;
+    unsigned tail = 0, head = 0;
+    for (int t = c.s; t < c.tb; ++t)
+        tail += (unsigned)__popcll(m[t]);
+    for (int t = c.ta; t < c.s; ++t)
+        head += (unsigned)__popcll(m[t]);
+    c.head = head;
+    c.pre = group_excl_scan<G>(tail + head, redu, c.tot);
+    return c;
+}
+
+// The exclusive prefix at every entry of the lane's chunk, out(t, prefix), in the scan's
+// visiting order.  Each entry is read before out is called for it, so out may overwrite
+// m[t] itself.
+template <typename Out>
+DEVI void chunk_offsets(const u64 *m, const ChunkScan &c, Out out)
+{
+    unsigned p = c.pre + c.head;
+    for (int t = c.s; t < c.tb; ++t) {
+        const u64 w = m[t];
+        out(t, p);
+        p += (unsigned)__popcll(w);
+    }
+    p = c.pre;
+    for (int t = c.ta; t < c.s; ++t) {
+        const u64 w = m[t];
+        out(t, p);
+        p += (unsigned)__popcll(w);
+    }
 }
 
 // ---- numeric forms ---------------------------------------------------------------------
@@ -351,29 +386,47 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
         else
             or_tiles<G, OR_FLAT, LIST>(op, t0, tlen, nA, lidx, nlong, dm, nullptr, nullptr, mn, tshift);
         __syncthreads();
-        int ta, tb;
-        unsigned tot;
-        unsigned pre = scan_popcounts<G>(dm, W, redu, tot, ta, tb);
+        const ChunkScan cs = scan_popcounts<G>(dm, W, redu);
+        const unsigned tot = cs.tot;
         const long long rd = (row >= 0 && op.dir_mask) ? op.rdir[row] : -1;
+        // Kept directories of groups of 256 lanes or more (chunks of many entries) go out
+        // coalesced: the masks first, then each entry's slot offset (flat rows: written
+        // over its mask in LDS; coarse level: into doff, which the fine pass reads) after a
+        // barrier.  keep is uniform over the launch, so every thread takes the barriers.
+        // Smaller groups store each entry from the chunk walk.
+        constexpr bool COAL = G >= 256;
+        const bool keep = COAL && op.dir_mask != nullptr;
         unsigned nnz = tot;
-        if (!sparse) {
-            if (rd >= 0) {
-                for (int t = ta; t < tb; ++t) {
-                    const u64 m = dm[t];
-                    op.dir_mask[rd + t] = m;
-                    op.dir_off[rd + t] = pre;
-                    pre += (unsigned)__popcll(m);
-                }
-            }
-        } else {
-            for (int t = ta; t < tb; ++t) {
-                const u64 m = dm[t];
-                doff[t] = pre;
+        if (COAL && !sparse && rd >= 0)
+            for (int t = l; t < W; t += G)
+                op.dir_mask[rd + t] = dm[t];
+        if (keep)
+            __syncthreads();
+        if (!COAL) {
+            chunk_offsets(dm, cs, [&](int t, unsigned p) {
+                if (sparse)
+                    doff[t] = p;
                 if (rd >= 0) {
-                    op.dir_mask[rd + t] = m;
-                    op.dir_off[rd + t] = pre;
+                    op.dir_mask[rd + t] = dm[t];
+                    op.dir_off[rd + t] = p;
                 }
-                pre += (unsigned)__popcll(m);
+            });
+        } else if (sparse) {
+            chunk_offsets(dm, cs, [&](int t, unsigned p) { doff[t] = p; });
+        } else if (rd >= 0) {
+            chunk_offsets(dm, cs, [&](int t, unsigned p) { dm[t] = p; });
+        }
+        if (keep)
+            __syncthreads();
+        if (COAL && rd >= 0) {
+            if (sparse) {
+                for (int t = l; t < W; t += G) {
+                    op.dir_mask[rd + t] = dm[t];
+                    op.dir_off[rd + t] = doff[t];
+                }
+            } else {
+                for (int t = l; t < W; t += G)
+                    op.dir_off[rd + t] = (unsigned)dm[t];
             }
         }
         if (two_level) {
@@ -387,19 +440,30 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
             if (sparse)
                 or_tiles<G, OR_FINE, LIST>(op, t0, tlen, nA, lidx, nlong, dm, doff, fm, mn, tshift);
             __syncthreads();
-            unsigned tot2;
-            unsigned pre2 = scan_popcounts<G>(fm, sparse ? (int)tot : 0, redu, tot2, ta, tb);
-            if (sparse) {
-                nnz = tot2;
-                if (rd >= 0) {
-                    for (int k = ta; k < tb; ++k) {
-                        const u64 m = fm[k];
-                        op.dir_mask[rd + W + k] = m;
-                        op.dir_off[rd + W + k] = pre2;
-                        pre2 += (unsigned)__popcll(m);
-                    }
-                }
+            const int F = sparse ? (int)tot : 0;
+            const ChunkScan fs = scan_popcounts<G>(fm, F, redu);
+            if (sparse)
+                nnz = fs.tot;
+            // the fine level goes out as the flat directory does
+            if (COAL && rd >= 0)
+                for (int k = l; k < F; k += G)
+                    op.dir_mask[rd + W + k] = fm[k];
+            if (keep)
+                __syncthreads();
+            if (rd >= 0) {
+                if (COAL)
+                    chunk_offsets(fm, fs, [&](int k, unsigned p) { fm[k] = p; });
+                else
+                    chunk_offsets(fm, fs, [&](int k, unsigned p) {
+                        op.dir_mask[rd + W + k] = fm[k];
+                        op.dir_off[rd + W + k] = p;
+                    });
             }
+            if (keep)
+                __syncthreads();
+            if (COAL && rd >= 0)
+                for (int k = l; k < F; k += G)
+                    op.dir_off[rd + W + k] = (unsigned)fm[k];
         }
         if (row >= 0 && l == 0)
             rnnz[row] = (int)nnz;
@@ -436,17 +500,11 @@ __global__ __launch_bounds__(1024) void k_symbolic_global(const int *__restrict_
                 atomicOr(&dm[op.tcol[t] - mn], op.tmask[t]);
         }
         block_sync_global();
-        int ta, tb;
-        unsigned tot;
-        unsigned pre = scan_popcounts<1024>(dm, S, red, tot, ta, tb);
-        if (rd >= 0) {
-            for (int t = ta; t < tb; ++t) {
-                op.dir_off[rd + t] = pre;
-                pre += (unsigned)__popcll(dm[t]);
-            }
-        }
+        const ChunkScan cs = scan_popcounts<1024>(dm, S, red);
+        if (rd >= 0)
+            chunk_offsets(dm, cs, [&](int t, unsigned p) { op.dir_off[rd + t] = p; });
         if (l == 0)
-            rnnz[row] = (int)tot;
+            rnnz[row] = (int)cs.tot;
         block_sync_global();
     }
 }
@@ -554,19 +612,15 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
         }
         // first level: flat slot offsets (rebuilt) or coarse compact indices (rebuilt), and
         // the total (flat: nnz(C_i); two-level: non-empty tiles)
-        int ta = 0, tb = 0;
         unsigned tot;
         if (known) {
             tot = sparse ? (W > 0 ? doff[W - 1] + (unsigned)__popcll(dm[W - 1]) : 0u) : (unsigned)n;
         } else {
             __syncthreads();
-            unsigned pre = scan_popcounts<G>(dm, W, redu, tot, ta, tb);
-            if (rd < 0) {
-                for (int t = ta; t < tb; ++t) {
-                    doff[t] = pre;
-                    pre += (unsigned)__popcll(dm[t]);
-                }
-            }
+            const ChunkScan cs = scan_popcounts<G>(dm, W, redu);
+            tot = cs.tot;
+            if (rd < 0)
+                chunk_offsets(dm, cs, [&](int t, unsigned p) { doff[t] = p; });
         }
         unsigned nnz = tot;
         const int F = sparse ? (int)tot : S; // fine entries
@@ -600,16 +654,11 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
                     nnz = (unsigned)n;
             } else {
                 __syncthreads();
-                unsigned tot2;
-                unsigned pre2 = scan_popcounts<G>(fm, sparse ? F : 0, redu, tot2, ta, tb);
+                const ChunkScan fs = scan_popcounts<G>(fm, sparse ? F : 0, redu);
                 if (sparse) {
-                    nnz = tot2;
-                    if (rd < 0) {
-                        for (int k = ta; k < tb; ++k) {
-                            foff[k] = pre2;
-                            pre2 += (unsigned)__popcll(fm[k]);
-                        }
-                    }
+                    nnz = fs.tot;
+                    if (rd < 0)
+                        chunk_offsets(fm, fs, [&](int k, unsigned p) { foff[k] = p; });
                 }
             }
         }
