@@ -59,7 +59,14 @@ class Pool {
     {
         if (bytes == 0)
             bytes = 1;
-        const size_t gran = bytes < (size_t(1) << 20) ? 256 : (size_t(2) << 20);
+        size_t gran = bytes < (size_t(1) << 20) ? 256 : (size_t(2) << 20);
+        // Blocks of a GiB or more come in eighth-octave sizes: C of a streamed sequence of
+        // row panels grows by fractions of a percent from panel to panel, and exact sizes
+        // made each slightly larger panel allocate a new set of tens of GB, keeping the
+        // old ones cached.  (Such allocations stalled for seconds once device memory had
+        // been freed in bulk.)  At most 12.5% of a block is unused.
+        if (bytes >= (size_t(1) << 30))
+            gran = size_t(1) << (63 - __builtin_clzll((unsigned long long)bytes) - 3);
         const size_t sz = (bytes + gran - 1) / gran * gran;
         auto it = free_.lower_bound(sz);
         if (it != free_.end() && it->first <= 2 * sz) {
