@@ -509,6 +509,51 @@ __global__ __launch_bounds__(1024) void k_symbolic_global(const int *__restrict_
     }
 }
 
+// Rows of the global bin whose flat directory alone fits LDS (their staged A entries did
+// not): the directory in LDS, B_j's tiles read from global memory per entry, as in
+// k_symbolic_global, but ORed with ds_or_b64 instead of L2 atomics.  A kept directory
+// goes out coalesced (masks, then the offsets written over them in LDS).  One 1024-lane
+// workgroup per row; dynamic LDS: 8 bytes per tile of the widest span.
+__global__ __launch_bounds__(1024) void k_symbolic_wide(const int *__restrict__ rows, const int *__restrict__ count_p,
+                                                        Operands op, int *__restrict__ rnnz, int tshift)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ unsigned red[16];
+    u64 *dm = (u64 *)lds;
+    const int l = threadIdx.x;
+    const int nrows = *count_p;
+    const int SG = 1 << tshift, sl = l & (SG - 1), sg = l >> tshift, NSG = 1024 >> tshift;
+    for (int idx = blockIdx.x; idx < nrows; idx += gridDim.x) {
+        const int row = uniform(rows[idx]);
+        const int mn = uniform(op.rmin[row]);
+        const int S = uniform(op.rmax[row]) - mn + 1;
+        const long long rd = op.dir_mask ? uniform64(op.rdir[row]) : -1;
+        for (int t = l; t < S; t += 1024)
+            dm[t] = 0;
+        __syncthreads();
+        const int a1 = op.Arp[row + 1];
+        for (int e = op.Arp[row] + sg; e < a1; e += NSG) {
+            const int j = op.Acol[e];
+            or_walk<OR_FLAT>(op, op.tptr[j], op.tptr[j + 1], SG, sl, dm, nullptr, nullptr, mn);
+        }
+        __syncthreads();
+        const ChunkScan cs = scan_popcounts<1024>(dm, S, red);
+        if (rd >= 0)
+            for (int t = l; t < S; t += 1024)
+                op.dir_mask[rd + t] = dm[t];
+        __syncthreads();
+        if (rd >= 0)
+            chunk_offsets(dm, cs, [&](int t, unsigned p) { dm[t] = p; });
+        __syncthreads();
+        if (rd >= 0)
+            for (int t = l; t < S; t += 1024)
+                op.dir_off[rd + t] = (unsigned)dm[t];
+        if (l == 0)
+            rnnz[row] = (int)cs.tot;
+        __syncthreads();
+    }
+}
+
 // ---- numeric -------------------------------------------------------------------------------
 
 // The HSMU accumulator in LDS (slice layout: row_layout).  The row's tile directory is
@@ -1741,14 +1786,19 @@ static void symbolic_l(int bin, const int *rows, const int *count_dev, int count
 
 hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
                            int *rnnz, int tshift, int two_level, int long_rows, u64 *gws, long long gws_stride,
-                           int num_cus, int gws_blocks, unsigned *queue, hipStream_t s)
+                           int num_cus, int gws_blocks, int lds_dyn, unsigned *queue, hipStream_t s)
 {
     if (count <= 0)
         return hipSuccess;
     if (bin == BIN_GLOBAL) {
         const int blocks = count < gws_blocks ? count : gws_blocks;
-        hipLaunchKernelGGL(k_symbolic_global, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, rnnz,
-                           tshift > 10 ? 10 : tshift, gws, gws_stride);
+        const size_t dir_lds = (size_t)gws_stride * sizeof(u64); // >= 8 bytes per tile of the widest span
+        if (dir_lds <= (size_t)lds_dyn)
+            hipLaunchKernelGGL(k_symbolic_wide, dim3(blocks), dim3(1024), dir_lds, s, rows, count_dev, op, rnnz,
+                               tshift > 10 ? 10 : tshift);
+        else
+            hipLaunchKernelGGL(k_symbolic_global, dim3(blocks), dim3(1024), 0, s, rows, count_dev, op, rnnz,
+                               tshift > 10 ? 10 : tshift, gws, gws_stride);
     } else if (bin >= 0 && bin < NLDS) {
         if (long_rows)
             symbolic_l<true>(bin, rows, count_dev, count, max_need, op, rnnz, tshift, two_level, num_cus, queue, s);
@@ -1884,6 +1934,7 @@ hipError_t configure_kernels(int lds_max)
     CFGP(PrecF64, false) CFGP(PrecF64, true) CFGP(PrecF32C, false) CFGP(PrecF32C, true)
 #undef CFGP
 #undef CFGL
+    CFG(k_symbolic_wide)
     CFG(k_numeric_window<PrecF64>) CFG(k_numeric_window<PrecF32C>)
     CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>) CFG(k_numeric_wsplit<PrecF64>) CFG(k_numeric_wsplit<PrecF32C>)
 #undef CFG

@@ -755,7 +755,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             continue;
         HIP_TRY(launch_symbolic(b, binrows + (size_t)b * M, &dst->sym.count[b], cnt, h1.sym.max_need[b], op, rnnz,
                                 tshift, h1.sym.nsparse[b] > 0 ? 1 : 0, long_rows, gws, gws_stride, c->num_cus,
-                                gws_blocks,
+                                gws_blocks, (int)bp.lds_row_max,
                                 opts.persistent ? dst->queue[0][b] : nullptr, s));
     }
     // Arrays of C handed to the caller; dropped again (one-shot calls) on any failure.

@@ -255,10 +255,12 @@ struct Operands {
 // queue: nullptr = one workgroup per row group (grid from count); otherwise 8 zeroed
 // counters and the LDS kernels run persistent (grid from occupancy x num_cus).
 // two_level: some row of the bin has a two-level directory (BinStats::nsparse > 0).
+// lds_dyn: LDS a row may use (lds_row_max); BIN_GLOBAL rows keep their flat directory in
+// LDS when gws_stride masks fit it (gws is then unused).
 hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int count,
                            int max_need, const Operands &op, int *rnnz, int tshift, int two_level, int long_rows,
                            unsigned long long *gws, long long gws_stride, int num_cus,
-                           int gws_blocks, unsigned *queue, hipStream_t s);
+                           int gws_blocks, int lds_dyn, unsigned *queue, hipStream_t s);
 
 hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *scan_ws,
                                long long *total_copy, hipStream_t s);
