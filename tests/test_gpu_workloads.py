@@ -91,3 +91,36 @@ def test_zipf_aat_panels(ctx, r0):
         dA.free()
         dB.free()
         ctx.trim()
+
+
+def test_zipf_panels_reuse_pool(ctx):
+    """Consecutive 2,000-row panels of the Zipf A*A^T: C of each is several GB and differs
+    from the last by a fraction of a percent.  Blocks of 1 GiB or more come in
+    eighth-octave sizes, so after the first panel the context's pool takes no further
+    block of C's size (smaller workspaces may still grow by a few MB)."""
+    A = hsmu.gen_zipf(500000, 100000, 8, 1.2, seed=4)
+    B = hsmu.transpose(A)
+    dA, dB = ctx.matrix(A), ctx.matrix(B)
+
+    def bucket(b):
+        g = 1 << (b.bit_length() - 4) if b >= 1 << 30 else 2 << 20
+        return -(-b // g) * g
+
+    try:
+        ctx.trim()
+        nnz, held = [], []
+        for p in range(4):
+            note(f"zipf pool reuse: panel rows [{2000 * p}, {2000 * (p + 1)})")
+            c, st = ctx.spgemm_rows_device(dA, 2000 * p, 2000 * (p + 1), dB, hsmu.default_opts(rowptr64=1))
+            nnz.append(int(c.nnz))
+            ctx.free_result(c)
+            held.append(hsmu.lib.hsmu_ctx_pool_bytes(ctx.handle))
+        note(f"nnz {nnz}, pool bytes {held}")
+        assert 4 * min(nnz) >= 1 << 30 and len(set(nnz)) > 1
+        # every panel's value and column arrays round to the first panel's block sizes
+        assert all(bucket(8 * n) == bucket(8 * nnz[0]) and bucket(4 * n) == bucket(4 * nnz[0]) for n in nnz)
+        assert held[-1] - held[0] < 1 << 30
+    finally:
+        dA.free()
+        dB.free()
+        ctx.trim()
