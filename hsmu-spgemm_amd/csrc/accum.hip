@@ -143,9 +143,10 @@ DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, c
 // Each lane scans a contiguous chunk [ta, tb) of the n directory entries; groups of 256
 // lanes or more visit it from s = ta + l mod chunk, wrapping round to ta: with every lane
 // starting at its chunk's first entry, a chunk of 16 entries would put all 64 lanes of a
-// wave on one LDS bank pair (smaller groups have short chunks and start at ta).  pre: the group's exclusive prefix of the popcounts at ta; head: the popcounts of
-// [ta, s); tot: the group total.  Every thread of the workgroup must call it (G > 64
-// synchronises the workgroup).
+// wave on one LDS bank pair.  Smaller groups have short chunks and walk them from ta
+// (s = ta) in one loop.  pre: the group's exclusive prefix of the popcounts at ta; head:
+// the popcounts of [ta, s); tot: the group total.  Every thread of the workgroup must
+// call it (G > 64 synchronises the workgroup).
 struct ChunkScan {
     int ta, tb, s;
     unsigned pre, head, tot;
@@ -158,36 +159,51 @@ DEVI ChunkScan scan_popcounts(const u64 *m, int n, unsigned *redu)
     const int chunk = (n + G - 1) / G;
     c.ta = l * chunk;
     c.tb = c.ta + chunk < n ? c.ta + chunk : n;
-    c.s = G >= 256 && chunk > 1 ? c.ta + l % chunk : c.ta;
-    if (c.s > c.tb)
-        c.s = c.tb;
     unsigned tail = 0, head = 0;
-    for (int t = c.s; t < c.tb; ++t)
-        tail += (unsigned)__popcll(m[t]);
-    for (int t = c.ta; t < c.s; ++t)
-        head += (unsigned)__popcll(m[t]);
+    if constexpr (G >= 256) {
+        c.s = chunk > 1 ? c.ta + l % chunk : c.ta;
+        if (c.s > c.tb)
+            c.s = c.tb;
+        for (int t = c.s; t < c.tb; ++t)
+            tail += (unsigned)__popcll(m[t]);
+        for (int t = c.ta; t < c.s; ++t)
+            head += (unsigned)__popcll(m[t]);
+    } else {
+        c.s = c.ta;
+        for (int t = c.ta; t < c.tb; ++t)
+            tail += (unsigned)__popcll(m[t]);
+    }
     c.head = head;
     c.pre = group_excl_scan<G>(tail + head, redu, c.tot);
     return c;
 }
 
-// The exclusive prefix at every entry of the lane's chunk, out(t, prefix), in the scan's
-// visiting order.  Each entry is read before out is called for it, so out may overwrite
-// m[t] itself.
-template <typename Out>
+// The exclusive prefix at every entry of the lane's chunk, out(t, prefix, m[t]), in the
+// scan's visiting order.  Each entry is read before out is called for it, so out may
+// overwrite m[t] itself.
+template <int G, typename Out>
 DEVI void chunk_offsets(const u64 *m, const ChunkScan &c, Out out)
 {
-    unsigned p = c.pre + c.head;
-    for (int t = c.s; t < c.tb; ++t) {
-        const u64 w = m[t];
-        out(t, p);
-        p += (unsigned)__popcll(w);
-    }
-    p = c.pre;
-    for (int t = c.ta; t < c.s; ++t) {
-        const u64 w = m[t];
-        out(t, p);
-        p += (unsigned)__popcll(w);
+    if constexpr (G >= 256) {
+        unsigned p = c.pre + c.head;
+        for (int t = c.s; t < c.tb; ++t) {
+            const u64 w = m[t];
+            out(t, p, w);
+            p += (unsigned)__popcll(w);
+        }
+        p = c.pre;
+        for (int t = c.ta; t < c.s; ++t) {
+            const u64 w = m[t];
+            out(t, p, w);
+            p += (unsigned)__popcll(w);
+        }
+    } else {
+        unsigned p = c.pre;
+        for (int t = c.ta; t < c.tb; ++t) {
+            const u64 w = m[t];
+            out(t, p, w);
+            p += (unsigned)__popcll(w);
+        }
     }
 }
 
@@ -403,18 +419,23 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
         if (keep)
             __syncthreads();
         if (!COAL) {
-            chunk_offsets(dm, cs, [&](int t, unsigned p) {
-                if (sparse)
+            if (sparse)
+                chunk_offsets<G>(dm, cs, [&](int t, unsigned p, u64 w) {
                     doff[t] = p;
-                if (rd >= 0) {
-                    op.dir_mask[rd + t] = dm[t];
+                    if (rd >= 0) {
+                        op.dir_mask[rd + t] = w;
+                        op.dir_off[rd + t] = p;
+                    }
+                });
+            else if (rd >= 0)
+                chunk_offsets<G>(dm, cs, [&](int t, unsigned p, u64 w) {
+                    op.dir_mask[rd + t] = w;
                     op.dir_off[rd + t] = p;
-                }
-            });
+                });
         } else if (sparse) {
-            chunk_offsets(dm, cs, [&](int t, unsigned p) { doff[t] = p; });
+            chunk_offsets<G>(dm, cs, [&](int t, unsigned p, u64) { doff[t] = p; });
         } else if (rd >= 0) {
-            chunk_offsets(dm, cs, [&](int t, unsigned p) { dm[t] = p; });
+            chunk_offsets<G>(dm, cs, [&](int t, unsigned p, u64) { dm[t] = p; });
         }
         if (keep)
             __syncthreads();
@@ -452,10 +473,10 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
                 __syncthreads();
             if (rd >= 0) {
                 if (COAL)
-                    chunk_offsets(fm, fs, [&](int k, unsigned p) { fm[k] = p; });
+                    chunk_offsets<G>(fm, fs, [&](int k, unsigned p, u64) { fm[k] = p; });
                 else
-                    chunk_offsets(fm, fs, [&](int k, unsigned p) {
-                        op.dir_mask[rd + W + k] = fm[k];
+                    chunk_offsets<G>(fm, fs, [&](int k, unsigned p, u64 w) {
+                        op.dir_mask[rd + W + k] = w;
                         op.dir_off[rd + W + k] = p;
                     });
             }
@@ -502,7 +523,7 @@ __global__ __launch_bounds__(1024) void k_symbolic_global(const int *__restrict_
         block_sync_global();
         const ChunkScan cs = scan_popcounts<1024>(dm, S, red);
         if (rd >= 0)
-            chunk_offsets(dm, cs, [&](int t, unsigned p) { op.dir_off[rd + t] = p; });
+            chunk_offsets<1024>(dm, cs, [&](int t, unsigned p, u64) { op.dir_off[rd + t] = p; });
         if (l == 0)
             rnnz[row] = (int)cs.tot;
         block_sync_global();
@@ -543,7 +564,7 @@ __global__ __launch_bounds__(1024) void k_symbolic_wide(const int *__restrict__ 
                 op.dir_mask[rd + t] = dm[t];
         __syncthreads();
         if (rd >= 0)
-            chunk_offsets(dm, cs, [&](int t, unsigned p) { dm[t] = p; });
+            chunk_offsets<1024>(dm, cs, [&](int t, unsigned p, u64) { dm[t] = p; });
         __syncthreads();
         if (rd >= 0)
             for (int t = l; t < S; t += 1024)
@@ -665,7 +686,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
             const ChunkScan cs = scan_popcounts<G>(dm, W, redu);
             tot = cs.tot;
             if (rd < 0)
-                chunk_offsets(dm, cs, [&](int t, unsigned p) { doff[t] = p; });
+                chunk_offsets<G>(dm, cs, [&](int t, unsigned p, u64) { doff[t] = p; });
         }
         unsigned nnz = tot;
         const int F = sparse ? (int)tot : S; // fine entries
@@ -703,7 +724,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
                 if (sparse) {
                     nnz = fs.tot;
                     if (rd < 0)
-                        chunk_offsets(fm, fs, [&](int k, unsigned p) { foff[k] = p; });
+                        chunk_offsets<G>(fm, fs, [&](int k, unsigned p, u64) { foff[k] = p; });
                 }
             }
         }
