@@ -92,15 +92,16 @@ DEVI void or_put(u64 *dm, const unsigned *coff, u64 *fm, int tt, u64 m)
         atomicOr(&fm[fine_index(dm, coff, tt)], m);
 }
 
-// Tiles [tb, te) of one B row by a group of gw lanes (this lane the gl-th).
-template <int MODE>
+// Tiles [tb, te) of one B row by a group of gw lanes (this lane the gl-th), U tiles per
+// lane per step.
+template <int MODE, int U = UNROLL>
 DEVI void or_walk(const Operands &op, int tb, int te, int gw, int gl, u64 *dm, const unsigned *coff, u64 *fm, int mn)
 {
-    for (int t = tb + gl; t < te; t += UNROLL * gw) {
-        int c[UNROLL];
-        u64 m[UNROLL];
+    for (int t = tb + gl; t < te; t += U * gw) {
+        int c[U];
+        u64 m[U];
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
+        for (int u = 0; u < U; ++u) {
             const int tu = t + u * gw;
             c[u] = -1;
             m[u] = 0;
@@ -111,7 +112,7 @@ DEVI void or_walk(const Operands &op, int tb, int te, int gw, int gl, u64 *dm, c
             }
         }
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u)
+        for (int u = 0; u < U; ++u)
             if (c[u] >= 0)
                 or_put<MODE>(dm, coff, fm, c[u] - mn, m[u]);
     }
@@ -124,11 +125,14 @@ DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, c
     const int l = threadIdx.x & (G - 1);
     // LIST: entries with many tiles (listed in lidx), one wave each (the whole group if
     // smaller); the sub-group walks below skip them
+    // 1024-lane groups hold a row per CU (LDS bounds their occupancy), so each lane keeps
+    // twice as many tile loads in flight
+    constexpr int U = G >= 1024 ? 2 * UNROLL : UNROLL;
     if constexpr (LIST) {
         constexpr int LW = G < 64 ? G : 64;
         for (int i = l / LW; i < nlong; i += G / LW) {
             const int e = lidx[i];
-            or_walk<MODE>(op, t0[e], t0[e] + tlen[e], LW, l & (LW - 1), dm, coff, fm, mn);
+            or_walk<MODE, U>(op, t0[e], t0[e] + tlen[e], LW, l & (LW - 1), dm, coff, fm, mn);
         }
     }
     // sub-groups of 2^tshift lanes take one entry each
@@ -136,7 +140,7 @@ DEVI void or_tiles(const Operands &op, const int *t0, const int *tlen, int nA, c
     for (int e = l >> tshift; e < nA; e += NSG) {
         if (LIST && tlen[e] >= LONG_TILES)
             continue;
-        or_walk<MODE>(op, t0[e], t0[e] + tlen[e], SG, sl, dm, coff, fm, mn);
+        or_walk<MODE, U>(op, t0[e], t0[e] + tlen[e], SG, sl, dm, coff, fm, mn);
     }
 }
 
@@ -555,7 +559,7 @@ __global__ __launch_bounds__(1024) void k_symbolic_wide(const int *__restrict__ 
         const int a1 = op.Arp[row + 1];
         for (int e = op.Arp[row] + sg; e < a1; e += NSG) {
             const int j = op.Acol[e];
-            or_walk<OR_FLAT>(op, op.tptr[j], op.tptr[j + 1], SG, sl, dm, nullptr, nullptr, mn);
+            or_walk<OR_FLAT, 2 * UNROLL>(op, op.tptr[j], op.tptr[j + 1], SG, sl, dm, nullptr, nullptr, mn);
         }
         __syncthreads();
         const ChunkScan cs = scan_popcounts<1024>(dm, S, red);
