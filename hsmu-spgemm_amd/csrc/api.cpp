@@ -734,14 +734,15 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             return fail(HSMU_ERR_NOMEM, "row directories (%llu entries)", dir_total);
     }
     // tile-mask workspace of the symbolic phase for rows that do not fit LDS when the
-    // directories are not kept: one span of masks per workgroup (widest span in the bin)
+    // directories are not kept: one span of masks per workgroup (widest span in the bin).
+    // Not needed when that span's directory fits the LDS a row may use (k_symbolic_wide).
     const int gws_tiles = h1.sym.max_span[BIN_GLOBAL] > 0 ? h1.sym.max_span[BIN_GLOBAL] : 1;
     const long long gws_stride = ((long long)gws_tiles + 31) & ~31LL; // u64 units, 256 B aligned
     int gws_blocks = 0;
     unsigned long long *gws = nullptr;
     if (h1.sym.count[BIN_GLOBAL] > 0) {
         gws_blocks = h1.sym.count[BIN_GLOBAL] < 2 * c->num_cus ? h1.sym.count[BIN_GLOBAL] : 2 * c->num_cus;
-        if (!keep_dirs) {
+        if (!keep_dirs && (unsigned long long)gws_stride * 8ull > (unsigned long long)bp.lds_row_max) {
             gws = ar.get<unsigned long long>((size_t)gws_stride * (size_t)gws_blocks);
             if (!gws)
                 return fail(HSMU_ERR_NOMEM, "symbolic tile-mask workspace");

@@ -169,8 +169,12 @@ def long_rows_pair(seed):
     return A, B
 
 
+# force_global: every row in the global bin, whose directories fit LDS (k_symbolic_wide);
+# lds_row_max=256 with rebuild_dirs: directories wider than 32 tiles in the global-memory
+# workspace (k_symbolic_global)
 @pytest.mark.parametrize("opts", [dict(), dict(force_global=1), dict(force_global=1, rebuild_dirs=1),
-                                  dict(lds_row_max=4096), dict(products_per_lane=512)])
+                                  dict(lds_row_max=256, rebuild_dirs=1), dict(lds_row_max=4096),
+                                  dict(products_per_lane=512)])
 def test_long_rows_of_B(ctx, opts):
     A, B = long_rows_pair(11)
     check(ctx, A, B, f"long rows of B {opts}", **opts)
