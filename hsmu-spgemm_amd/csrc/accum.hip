@@ -496,8 +496,9 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
     }
 }
 
-// Rows too wide for LDS: the flat directory over the row's span in global memory, updated
-// with L2 atomics -- in the kept directory (rdir) when the directories are kept, so the
+// Rows of the global bin whose flat directory does not fit LDS either (narrower ones run
+// in k_symbolic_wide): the directory over the row's span in global memory, updated with
+// L2 atomics -- in the kept directory (rdir) when the directories are kept, so the
 // numeric phase reads masks and slot offsets from there, else in a per-workgroup
 // workspace.  One 1024-lane workgroup per row.
 __global__ __launch_bounds__(1024) void k_symbolic_global(const int *__restrict__ rows, const int *__restrict__ count_p,
