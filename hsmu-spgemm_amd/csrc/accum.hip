@@ -1839,7 +1839,7 @@ hipError_t launch_symbolic(int bin, const int *rows, const int *count_dev, int c
 template <int G, typename P, bool LONG>
 static void numeric_g(const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
                       const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int two_level, int *err,
-                      int num_cus, unsigned *queue, hipStream_t s)
+                      int num_cus, unsigned *queue, unsigned long long *lds_bytes, hipStream_t s)
 {
     constexpr int BS = G < BLOCK ? BLOCK : G;
     constexpr int RPB = BS / G;
@@ -1847,6 +1847,7 @@ static void numeric_g(const int *rows, const int *count_dev, int count, int max_
     const size_t lds = (size_t)slice * RPB;
     const int64_t blocks =
         row_grid(k_numeric_lds<G, P, LONG>, BS, lds, ((int64_t)count + RPB - 1) / RPB, num_cus, queue);
+    *lds_bytes += (unsigned long long)blocks * lds;
     const int lg = __builtin_ctz(G);
     hipLaunchKernelGGL((k_numeric_lds<G, P, LONG>), dim3((unsigned)blocks), dim3(BS), lds, s, rows, count_dev, slice,
                        op, crp, Ccol, (typename P::V *)Cval, bshift > lg ? lg : bshift, tshift > lg ? lg : tshift,
@@ -1856,12 +1857,12 @@ static void numeric_g(const int *rows, const int *count_dev, int count, int max_
 template <typename P, bool LONG>
 static void numeric_l(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
                       const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int two_level, int *err,
-                      int num_cus, unsigned *queue, hipStream_t s)
+                      int num_cus, unsigned *queue, unsigned long long *lds_bytes, hipStream_t s)
 {
 #define NUM(B, G)                                                                                                      \
     case B:                                                                                                            \
         numeric_g<G, P, LONG>(rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level, err,   \
-                              num_cus, queue, s);                                                                      \
+                              num_cus, queue, lds_bytes, s);                                                           \
         break;
     switch (bin) {
         NUM(0, 8) NUM(1, 16) NUM(2, 32) NUM(3, 64) NUM(4, 128) NUM(5, 256) NUM(6, 512) NUM(7, 1024)
@@ -1872,10 +1873,13 @@ static void numeric_l(int bin, const int *rows, const int *count_dev, int count,
 template <typename P>
 static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int count, int max_need, const Operands &op,
                             const long long *crp, int *Ccol, void *Cval, int bshift, int tshift, int two_level,
-                            int long_rows, const WideRows &wide, int *err, int num_cus, unsigned *queue, hipStream_t s)
+                            int long_rows, const WideRows &wide, int *err, int num_cus, unsigned *queue,
+                            unsigned long long *lds_bytes, hipStream_t s)
 {
     if (bin == BIN_GLOBAL) {
         const int blocks = count < wide.blocks ? count : wide.blocks;
+        *lds_bytes += op.dir_mask && wide.items ? (unsigned long long)wide.blocks * WS_LDS
+                                                : (unsigned long long)blocks * (op.dir_mask ? CW_LDS : WIN_LDS);
         if (op.dir_mask && wide.items) { // split work items (FP64, or compensated with its scratch)
             constexpr bool F32C = std::is_same<P, PrecF32C>::value;
             hipError_t e = launch_fill_zero(wide.ctr, WIDE_CTRS * sizeof(unsigned), s);
@@ -1910,10 +1914,10 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
     } else if (bin >= 0 && bin < NLDS) {
         if (long_rows)
             numeric_l<P, true>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level,
-                               err, num_cus, queue, s);
+                               err, num_cus, queue, lds_bytes, s);
         else
             numeric_l<P, false>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level,
-                                err, num_cus, queue, s);
+                                err, num_cus, queue, lds_bytes, s);
     } else {
         return hipErrorInvalidValue;
     }
@@ -1923,15 +1927,18 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
 hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count, int max_need,
                           const Operands &op, const long long *crp, int *Ccol, void *Cval, int bshift, int tshift,
                           int two_level, int long_rows, const WideRows &wide, int *err, int num_cus, unsigned *queue,
-                          hipStream_t s)
+                          unsigned long long *lds_bytes, hipStream_t s)
 {
     if (count <= 0)
         return hipSuccess;
+    unsigned long long ignored = 0;
+    if (!lds_bytes)
+        lds_bytes = &ignored;
     if (fp32c)
         return numeric_p<PrecF32C>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level,
-                                   long_rows, wide, err, num_cus, queue, s);
+                                   long_rows, wide, err, num_cus, queue, lds_bytes, s);
     return numeric_p<PrecF64>(bin, rows, count_dev, count, max_need, op, crp, Ccol, Cval, bshift, tshift, two_level,
-                              long_rows, wide, err, num_cus, queue, s);
+                              long_rows, wide, err, num_cus, queue, lds_bytes, s);
 }
 
 size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks)

@@ -109,6 +109,17 @@ class Pool {
         free_.clear();
     }
 
+    // Free every block, cached or still handed out (results the caller did not release).
+    void release_all()
+    {
+        trim();
+        for (auto &kv : used_) {
+            (void)hipFree(kv.first);
+            held_ -= kv.second;
+        }
+        used_.clear();
+    }
+
     size_t held() const { return held_; }
 
   private:
@@ -217,16 +228,19 @@ int hsmu_ctx_create(int device, hsmu_ctx **out)
     c->num_cus = prop.multiProcessorCount;
     c->lds_max = (int)prop.sharedMemPerBlock;
     c->total_mem = prop.totalGlobalMem;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void **)&c->host_stats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) {
-        delete c;
-        return fail(HSMU_ERR_HIP, "hsmu_ctx_create: stream or pinned memory");
+    // every failure from here on releases what was already created
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipHostMalloc((void **)&c->host_stats, sizeof(DevStats), hipHostMallocDefault) == hipSuccess;
+    for (int i = 0; ok && i < EV_COUNT; ++i)
+        ok = hipEventCreate(&c->ev[i]) == hipSuccess;
+    for (int b = 0; ok && b < NBINS; ++b)
+        for (int k = 0; ok && k < 2; ++k)
+            ok = hipEventCreate(&c->evb[b][k]) == hipSuccess;
+    if (!ok) {
+        const hipError_t e = hipGetLastError();
+        hsmu_ctx_destroy(c);
+        return fail(HSMU_ERR_HIP, "hsmu_ctx_create: stream, pinned memory or events: %s", hipGetErrorString(e));
     }
-    for (int i = 0; i < EV_COUNT; ++i)
-        HIP_TRY(hipEventCreate(&c->ev[i]));
-    for (int b = 0; b < NBINS; ++b)
-        for (int k = 0; k < 2; ++k)
-            HIP_TRY(hipEventCreate(&c->evb[b][k]));
     static std::mutex mu;
     static std::vector<int> configured;
     {
@@ -254,7 +268,7 @@ void hsmu_ctx_destroy(hsmu_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
-    c->pool.trim();
+    c->pool.release_all();
     for (int i = 0; i < EV_COUNT; ++i)
         if (c->ev[i])
             (void)hipEventDestroy(c->ev[i]);
@@ -819,6 +833,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
 
     // ---- 5. numeric ----
     int *errp = &dst->err;
+    unsigned long long lds_launched = 0;
     for (int b = 0; b < NBINS; ++b) {
         const int cnt = h2.num.count[b];
         if (cnt <= 0)
@@ -860,7 +875,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         HIP_TRY(launch_numeric(b, fp32 ? 1 : 0, binrows + (size_t)b * M, &dst->num.count[b], cnt, h2.num.max_need[b], op,
                                crp, Ccol, Cval, bshift, tshift, h2.num.nsparse[b] > 0 ? 1 : 0, long_rows, wide, errp,
                                c->num_cus,
-                               opts.persistent ? dst->queue[1][b] : nullptr, s));
+                               opts.persistent ? dst->queue[1][b] : nullptr, &lds_launched, s));
         if (timing)
             HIP_TRY(hipEventRecord(c->evb[b][1], s));
     }
@@ -931,6 +946,14 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             ((double)(K + 1) * 4.0 + (double)h2.ntiles_b * (4.0 + 8.0)) / (1024.0 * 1024.0);
         S.alg_bytes = 4.0 * (double)(M + 1) + (12.0 + vsz) * (double)P.nnzA + (4.0 + vsz) * (double)S.products +
                       (rp64 ? 8.0 : 4.0) * (double)(M + 1) + (4.0 + vsz) * (double)nnzc;
+        // compulsory traffic: every input array read once and C written once (A and B the
+        // same arrays for a self-product A*A)
+        const bool self = P.Arp == P.Brp && P.Acol == P.Bcol && P.Aval == P.Bval;
+        S.compulsory_bytes = (4.0 * (double)(M + 1) + (4.0 + vsz) * (double)P.nnzA) +
+                             (self ? 0.0 : 4.0 * (double)(K + 1) + (4.0 + vsz) * (double)P.nnzB) +
+                             (rp64 ? 8.0 : 4.0) * (double)(M + 1) + (4.0 + vsz) * (double)nnzc;
+        S.lds_launched = (double)lds_launched;
+        S.lds_use = lds_launched ? 12.0 * (double)nnzc / (double)lds_launched : 0.0;
     }
     return HSMU_OK;
 }
@@ -1316,6 +1339,28 @@ extern "C" int hsmu_spgemm_batched(hsmu_ctx *c, int64_t count, const hsmu_csr *A
     Arena ar(c, nullptr, false);
     bool empty;
     return run_batch(c, count, A, B, C, opts, ar, RunMode(), stats, &empty);
+}
+
+extern "C" int hsmu_spgemm_each(hsmu_ctx *c, int64_t count, const hsmu_csr *A, const hsmu_csr *B,
+                                const hsmu_opts *opts, int32_t passes, double *ms_per_pass)
+{
+    if (!c || !A || !B || count < 1 || passes < 1 || !ms_per_pass)
+        return fail(HSMU_ERR_INVALID, "hsmu_spgemm_each: NULL argument, count < 1 or passes < 1");
+    HIP_TRY(hipSetDevice(c->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int32_t p = 0; p < passes; ++p) {
+        for (int64_t i = 0; i < count; ++i) {
+            hsmu_csr C;
+            int rc = hsmu_spgemm(c, A + i, B + i, &C, opts, nullptr);
+            if (rc)
+                return fail(rc, "hsmu_spgemm_each: instance %lld: %s", (long long)i, hsmu_last_error());
+            rc = hsmu_csr_free(c, &C);
+            if (rc)
+                return rc;
+        }
+    }
+    *ms_per_pass = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / passes;
+    return HSMU_OK;
 }
 
 extern "C" int hsmu_batch_free(hsmu_ctx *c, int64_t count, hsmu_csr *C)

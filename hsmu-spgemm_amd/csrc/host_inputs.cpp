@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -396,11 +397,8 @@ int hsmu_transpose(const hsmu_host_csr *A, hsmu_host_csr *AT)
 // entries get 1.0, complex keeps the real part), symmetric and hermitian files are
 // expanded by mirroring every off-diagonal entry, duplicates are kept.  Each row is then
 // stable-sorted by column, which the reference leaves to the file.
-int hsmu_read_mtx(const char *path, hsmu_host_csr *out, int32_t *is_symmetric)
+static int read_mtx(const char *path, hsmu_host_csr *out, int32_t *is_symmetric)
 {
-    if (!path || !out)
-        return HSMU_ERR_INVALID;
-    memset(out, 0, sizeof(*out));
     FILE *f = fopen(path, "rb");
     if (!f)
         return HSMU_ERR_INVALID;
@@ -442,6 +440,13 @@ int hsmu_read_mtx(const char *path, hsmu_host_csr *out, int32_t *is_symmetric)
     p = q;
     if (m < 0 || n < 0 || nz < 0 || m >= INT32_MAX || n >= INT32_MAX)
         return HSMU_ERR_UNSUPPORTED;
+    // every entry takes at least four bytes of the file ("1 1\n"): a larger count is a
+    // truncated or corrupt file, not a reason to reserve memory for it
+    if (nz > (long long)(end - p) / 4 + 1)
+        return HSMU_ERR_INVALID;
+    // mirrored entries land in row c - 1 < n, so a symmetric file must be square
+    if (symmetric && m != n)
+        return HSMU_ERR_INVALID;
     std::vector<int32_t> ri((size_t)nz), ci((size_t)nz);
     std::vector<double> vi((size_t)nz);
     for (long long k = 0; k < nz; ++k) {
@@ -509,6 +514,24 @@ int hsmu_read_mtx(const char *path, hsmu_host_csr *out, int32_t *is_symmetric)
         }
     }
     return HSMU_OK;
+}
+
+int hsmu_read_mtx(const char *path, hsmu_host_csr *out, int32_t *is_symmetric)
+{
+    if (!path || !out)
+        return HSMU_ERR_INVALID;
+    memset(out, 0, sizeof(*out));
+    int rc;
+    try {
+        rc = read_mtx(path, out, is_symmetric);
+    } catch (const std::bad_alloc &) {
+        rc = HSMU_ERR_NOMEM;
+    } catch (...) {
+        rc = HSMU_ERR_INVALID;
+    }
+    if (rc != HSMU_OK)
+        hsmu_host_csr_free(out);
+    return rc;
 }
 
 } // extern "C"

@@ -316,12 +316,14 @@ struct WideRows {
 size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks);
 
 // Numeric phase of one bin.  fp32c = 0: FP64 values (Cval double); fp32c = 1: compensated
-// FP32 (Cval float).
+// FP32 (Cval float).  lds_bytes (may be null): += the dynamic LDS of the launch, workgroups
+// x bytes per workgroup (the reference's "practical need", compressed_Cval.cuh:202-533).
 hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count,
                           int max_need, const Operands &op, const long long *crp, int *Ccol,
                           void *Cval, int bshift, int tshift, int two_level, int long_rows,
                           const WideRows &wide,
-                          int *err, int num_cus, unsigned *queue, hipStream_t s);
+                          int *err, int num_cus, unsigned *queue, unsigned long long *lds_bytes,
+                          hipStream_t s);
 
 hipError_t launch_rowptr_i64_to_i32(const long long *in, int64_t n, int *out, hipStream_t s);
 

@@ -80,7 +80,8 @@ class Stats(C.Structure):
                 ("tiles_b", C.c_int64), ("rows_nonempty", C.c_int64), ("sym_bin_rows", C.c_int64 * NBINS),
                 ("num_bin_rows", C.c_int64 * NBINS), ("compress_rate", C.c_double), ("gflops", C.c_double),
                 ("space_compressed_mask_mb", C.c_double), ("alg_bytes", C.c_double),
-                ("num_bin_bytes", C.c_double * NBINS), ("num_bin_products", C.c_int64 * NBINS)]
+                ("num_bin_bytes", C.c_double * NBINS), ("num_bin_products", C.c_int64 * NBINS),
+                ("compulsory_bytes", C.c_double), ("lds_launched", C.c_double), ("lds_use", C.c_double)]
 
     def to_dict(self) -> dict:
         out = {}
@@ -101,6 +102,7 @@ _sig = {
     "hsmu_ctx_num_cus": (C.c_int, [_P]),
     "hsmu_opts_default": (None, [C.POINTER(Opts)]),
     "hsmu_status_string": (C.c_char_p, [C.c_int]),
+    "hsmu_build_info": (C.c_char_p, []),
     "hsmu_last_error": (C.c_char_p, []),
     "hsmu_spgemm": (C.c_int, [_P, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Opts),
                               C.POINTER(Stats)]),
@@ -111,6 +113,8 @@ _sig = {
     "hsmu_spgemm_batched": (C.c_int, [_P, C.c_int64, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr),
                                       C.POINTER(Opts), C.POINTER(Stats)]),
     "hsmu_batch_free": (C.c_int, [_P, C.c_int64, C.POINTER(Csr)]),
+    "hsmu_spgemm_each": (C.c_int, [_P, C.c_int64, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Opts), C.c_int32,
+                                   C.POINTER(C.c_double)]),
     "hsmu_plan_create": (C.c_int, [_P, C.c_int64, C.POINTER(Csr), C.POINTER(Csr), C.POINTER(Csr),
                                    C.POINTER(Opts), C.POINTER(_P)]),
     "hsmu_plan_execute": (C.c_int, [_P, C.POINTER(Stats)]),
@@ -141,6 +145,36 @@ for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)
     _f.restype = _res
     _f.argtypes = _args
+
+
+def build_info() -> str:
+    """The loaded library's stamp: "head=<commit>[+dirty] src=<hash>"."""
+    return lib.hsmu_build_info().decode()
+
+
+def source_hash() -> str:
+    """The hash the Makefile stamps into libhsmu.so, recomputed from the sources in this
+    tree: SHA-256 of csrc/*.hip, *.cpp, *.h and include/*.h concatenated in sorted path
+    order (paths as the Makefile names them), first 16 hex digits."""
+    import glob
+    import hashlib
+    pkg = os.path.dirname(_HERE)
+    rel = []
+    for pat in ("csrc/*.hip", "csrc/*.cpp", "csrc/*.h", "../include/*.h"):
+        rel += [os.path.relpath(f, pkg) for f in glob.glob(os.path.join(pkg, pat))]
+    h = hashlib.sha256()
+    for r in sorted(rel):
+        with open(os.path.join(pkg, r), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def provenance() -> dict:
+    """The loaded library's stamp and whether its source hash matches this tree's."""
+    stamp = build_info()
+    src = stamp.split("src=")[-1]
+    here = source_hash()
+    return {"stamp": stamp, "tree_src": here, "src_match": src == here}
 
 
 def _check(status: int, where: str) -> None:
@@ -415,6 +449,15 @@ class Context:
         o = opts if opts is not None else default_opts()
         _check(lib.hsmu_spgemm_batched(self.handle, n, a, b, cs, C.byref(o), C.byref(st)), "hsmu_spgemm_batched")
         return cs, st.to_dict()
+
+    def spgemm_each_ms(self, As, Bs, opts: Opts | None = None, passes: int = 1) -> float:
+        """ms per pass of count sequential hsmu_spgemm calls (C_i released at once),
+        timed inside the library around the whole loop (no Python per call)."""
+        n, a, b = self._pair_arrays(As, Bs)
+        ms = C.c_double(0)
+        o = opts if opts is not None else default_opts()
+        _check(lib.hsmu_spgemm_each(self.handle, n, a, b, C.byref(o), passes, C.byref(ms)), "hsmu_spgemm_each")
+        return ms.value
 
     def free_batch(self, cs) -> None:
         _check(lib.hsmu_batch_free(self.handle, len(cs), cs), "hsmu_batch_free")

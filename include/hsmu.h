@@ -161,6 +161,12 @@ typedef struct hsmu_stats {
     double alg_bytes;         /* algorithmic HBM bytes of the whole SpGEMM (gather model) */
     double num_bin_bytes[HSMU_NBINS];    /* the same model over each numeric bin's rows */
     int64_t num_bin_products[HSMU_NBINS];
+    double compulsory_bytes;  /* every input array read once, C written once: the lower bound
+                                 of HBM traffic */
+    double lds_launched;      /* dynamic LDS of the numeric launches, workgroups x bytes each
+                                 (the reference's practical_need, compressed_Cval.cuh:202-533) */
+    double lds_use;           /* nnz(C) * 12 / lds_launched: the reference's use_of_shared
+                                 (compressed_Cval.cuh:564-570) */
 } hsmu_stats;
 
 /* ---- context ------------------------------------------------------------------- */
@@ -170,6 +176,9 @@ typedef struct hsmu_ctx hsmu_ctx;
 /* Create a context on a device: one HIP stream (default priority), a caching device
  * allocator, pinned staging memory.  One context per host thread. */
 int hsmu_ctx_create(int device, hsmu_ctx **out);
+/* Destroy a context.  Every device block its pool holds is freed, including results
+ * (hsmu_csr, batches) that were not released: release results and destroy plans BEFORE
+ * their context; releasing them afterwards is invalid. */
 void hsmu_ctx_destroy(hsmu_ctx *ctx);
 void *hsmu_ctx_stream(hsmu_ctx *ctx);       /* the hipStream_t, as void* */
 int hsmu_ctx_trim(hsmu_ctx *ctx);           /* release cached device memory */
@@ -179,6 +188,9 @@ int hsmu_ctx_num_cus(hsmu_ctx *ctx);
 
 void hsmu_opts_default(hsmu_opts *opts);
 const char *hsmu_status_string(int status);
+/* "head=<commit>[+dirty] src=<hash>": the commit and the source hash this library was
+ * built from (no counterpart in the reference). */
+const char *hsmu_build_info(void);
 const char *hsmu_last_error(void);          /* message of the last failure on this thread */
 
 /* ---- the operator ---------------------------------------------------------------- */
@@ -222,6 +234,13 @@ int hsmu_tiles_free(hsmu_ctx *ctx, hsmu_tiles *T);
 int hsmu_spgemm_batched(hsmu_ctx *ctx, int64_t count, const hsmu_csr *A, const hsmu_csr *B,
                         hsmu_csr *C, const hsmu_opts *opts, hsmu_stats *stats);
 int hsmu_batch_free(hsmu_ctx *ctx, int64_t count, hsmu_csr *C);
+
+/* The baseline the batched form is measured against: `passes` times, C = A[i] * B[i] for
+ * every i by one hsmu_spgemm call each (each result released right away), timed on the
+ * host around the whole loop; *ms_per_pass = the time of one pass over the count
+ * instances.  No counterpart in the reference (one product per process, src/test.cu). */
+int hsmu_spgemm_each(hsmu_ctx *ctx, int64_t count, const hsmu_csr *A, const hsmu_csr *B,
+                     const hsmu_opts *opts, int32_t passes, double *ms_per_pass);
 
 /* ---- plans: inspect once, replay as one HIP graph ----------------------------------
  * hsmu_plan_create runs the batch (count >= 1; count == 1 is a single SpGEMM) once,

@@ -156,3 +156,23 @@ def test_read_mtx_rejects_bad_index(tmp_path):
     p.write_text("%%MatrixMarket matrix coordinate real general\n2 2 1\n3 1 1.0\n")
     with pytest.raises(hsmu.HsmuError):
         hsmu.read_mtx(str(p))
+
+
+def test_read_mtx_rejects_malformed_headers(tmp_path):
+    """A symmetric file must be square (its mirrored entries land in rows up to N), and an
+    entry count larger than the file can hold is a corrupt file: both are errors, never an
+    out-of-bounds write or an allocation sized by the header."""
+    p = tmp_path / "ns.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real symmetric\n2 40000 2\n1 1 1.0\n2 39999 2.0\n")
+    with pytest.raises(hsmu.HsmuError) as e:
+        hsmu.read_mtx(str(p))
+    assert e.value.status == hsmu.HSMU_ERR_INVALID
+    q = tmp_path / "big.mtx"
+    q.write_text("%%MatrixMarket matrix coordinate real general\n4 4 9000000000000000\n1 1 1.0\n")
+    with pytest.raises(hsmu.HsmuError) as e:
+        hsmu.read_mtx(str(q))
+    assert e.value.status == hsmu.HSMU_ERR_INVALID
+    r = tmp_path / "sq.mtx"
+    r.write_text("%%MatrixMarket matrix coordinate real symmetric\n3 3 2\n1 1 1.0\n3 2 2.0\n")
+    m, sym = hsmu.read_mtx(str(r))
+    assert sym and m.nnz == 3 and m.rowptr.tolist() == [0, 1, 2, 3] and m.col.tolist() == [0, 2, 1]
