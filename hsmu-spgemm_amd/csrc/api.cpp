@@ -180,6 +180,7 @@ struct hsmu_ctx {
     int num_cus = 0;
     int lds_max = 0;
     size_t total_mem = 0; // device memory (bytes)
+    void *scratch = nullptr; // device address of hsmu_spgemm_noalloc's array on this device
 };
 
 extern "C" {
@@ -240,6 +241,12 @@ int hsmu_ctx_create(int device, hsmu_ctx **out)
         const hipError_t e = hipGetLastError();
         hsmu_ctx_destroy(c);
         return fail(HSMU_ERR_HIP, "hsmu_ctx_create: stream, pinned memory or events: %s", hipGetErrorString(e));
+    }
+    // the allocation-free entry point's workspace on this device (looked up once per
+    // device); without it only that entry point is unavailable
+    if (scratch_setup(device, &c->scratch) != hipSuccess) {
+        (void)hipGetLastError();
+        c->scratch = nullptr;
     }
     static std::mutex mu;
     static std::vector<int> configured;
@@ -530,9 +537,27 @@ namespace {
 // workspace and outputs alike, until it is destroyed, and its capture pass is handed the
 // same blocks again in the same order (the sequence of requests depends only on the
 // recorded checkpoints, which the capture pass reuses).
+// The caller's arrays for C in an allocation-free call: row pointers of M + 1 entries,
+// columns and values of cap_nnz entries each (vsz bytes per value).
+struct FixedOut {
+    void *rowptr;
+    int *col;
+    void *val;
+    int64_t cap_nnz;
+    size_t vsz;
+};
+
+enum OutRole { OUT_ROWPTR, OUT_COL, OUT_VAL };
+
 class Arena {
   public:
     Arena(hsmu_ctx *c, std::vector<void *> *keep, bool replay) : c_(c), keep_(keep), replay_(replay) {}
+    // Allocation-free: the workspace is carved from [base, base + size) (256-byte aligned
+    // pieces, nothing is allocated or freed) and C goes into the caller's arrays.
+    Arena(hsmu_ctx *c, char *base, size_t size, const FixedOut *fixed)
+        : c_(c), keep_(nullptr), replay_(false), base_(base), size_(size), fixed_(fixed)
+    {
+    }
     ~Arena()
     {
         for (void *p : taken_)
@@ -540,6 +565,16 @@ class Arena {
     }
     void *raw(size_t bytes)
     {
+        if (base_) {
+            const size_t o = (off_ + 255) & ~(size_t)255;
+            if (o > size_ || bytes > size_ - o) {
+                if (!short_)
+                    short_ = o + bytes;
+                return nullptr;
+            }
+            off_ = o + (bytes ? bytes : 1);
+            return base_ + o;
+        }
         if (replay_)
             return pos_ < keep_->size() ? (*keep_)[pos_++] : nullptr;
         void *p = c_->pool.alloc(bytes);
@@ -548,14 +583,34 @@ class Arena {
         return p;
     }
     template <typename T> T *get(size_t n) { return (T *)raw(n * sizeof(T)); }
+    // bytes a further workspace request may take (unbounded unless allocation-free)
+    size_t remaining() const { return base_ ? (off_ < size_ ? size_ - off_ : 0) : SIZE_MAX; }
+    bool fixed() const { return fixed_ != nullptr; }
     // an output array of a one-shot call (plain hipMalloc or the pool, not given back at
-    // exit); a plan's outputs are kept blocks like the rest
-    void *out(size_t bytes, bool plain) { return keep_ ? raw(bytes) : out_alloc(c_, bytes, plain); }
+    // exit); a plan's outputs are kept blocks like the rest; an allocation-free call's are
+    // the caller's arrays (nullptr when C's entries exceed their capacity)
+    void *out(size_t bytes, bool plain, OutRole role)
+    {
+        if (fixed_) {
+            if (role == OUT_ROWPTR)
+                return fixed_->rowptr;
+            const size_t esz = role == OUT_COL ? sizeof(int) : fixed_->vsz;
+            if (bytes > (size_t)fixed_->cap_nnz * esz) {
+                want_nnz_ = (int64_t)(bytes / esz);
+                return nullptr;
+            }
+            void *p = role == OUT_COL ? (void *)fixed_->col : fixed_->val;
+            return p ? p : base_; // nnz(C) = 0 with no arrays: any valid address, never written
+        }
+        return keep_ ? raw(bytes) : out_alloc(c_, bytes, plain);
+    }
     void drop_out(void *p)
     {
-        if (!keep_)
+        if (!keep_ && !fixed_)
             out_free(c_, p);
     }
+    int64_t want_nnz() const { return want_nnz_; }    // nnz(C) that exceeded the capacity
+    size_t short_bytes() const { return short_; }     // workspace end a failed request needed
 
   private:
     hsmu_ctx *c_;
@@ -563,6 +618,10 @@ class Arena {
     bool replay_;
     size_t pos_ = 0;
     std::vector<void *> taken_;
+    char *base_ = nullptr;
+    size_t size_ = 0, off_ = 0, short_ = 0;
+    const FixedOut *fixed_ = nullptr;
+    int64_t want_nnz_ = 0;
 };
 
 // How a pipeline run meets its two checkpoints: eagerly (copy the device counters to
@@ -736,7 +795,11 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             tile_products += h1.sym.work[b];
         const bool worth = 2 * dir_total <= tile_products || h1.sym.count[BIN_GLOBAL] > 0 ||
                            h1.num_wide_hint > 0;
-        const unsigned long long budget = c->total_mem / 4 > (1ull << 31) ? c->total_mem / 4 : (1ull << 31);
+        unsigned long long budget = c->total_mem / 4 > (1ull << 31) ? c->total_mem / 4 : (1ull << 31);
+        // allocation-free: at most half of the workspace left, the rest serves C's row
+        // pointers and the numeric phase
+        if (budget > ar.remaining() / 2)
+            budget = ar.remaining() / 2;
         keep_dirs = !opts.rebuild_dirs && worth && dir_total > 0 && dir_total * 12ull <= budget;
         if (mode.plan)
             mode.plan->dirs_kept = keep_dirs;
@@ -786,7 +849,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
     // C row pointers (int64); they are C's rowptr when a single product asks for int64
     long long *crp;
     if (rp64 && !batch) {
-        crp = (long long *)ar.out(((size_t)M + 1) * sizeof(long long), plain);
+        crp = (long long *)ar.out(((size_t)M + 1) * sizeof(long long), plain, OUT_ROWPTR);
         if (crp)
             outs.p.push_back(crp);
     } else {
@@ -808,21 +871,21 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
 
     // ---- 4. allocate C ----
     const auto t_alloc0 = std::chrono::steady_clock::now();
-    int *Ccol = (int *)ar.out((size_t)nnzc * sizeof(int), plain);
+    int *Ccol = (int *)ar.out((size_t)nnzc * sizeof(int), plain, OUT_COL);
     outs.p.push_back(Ccol);
-    void *Cval = ar.out((size_t)nnzc * (size_t)vsz, plain);
+    void *Cval = ar.out((size_t)nnzc * (size_t)vsz, plain, OUT_VAL);
     outs.p.push_back(Cval);
     void *Crp;
     long long *bases = nullptr;
     if (batch) {
-        Crp = ar.out((size_t)(M + batch->count) * (rp64 ? 8 : 4), plain);
+        Crp = ar.out((size_t)(M + batch->count) * (rp64 ? 8 : 4), plain, OUT_ROWPTR);
         outs.p.push_back(Crp);
         bases = ar.get<long long>((size_t)batch->count + 1);
         batch->bases_dev = bases;
     } else if (rp64) {
         Crp = crp;
     } else {
-        Crp = ar.out(((size_t)M + 1) * sizeof(int), plain);
+        Crp = ar.out(((size_t)M + 1) * sizeof(int), plain, OUT_ROWPTR);
         outs.p.push_back(Crp);
     }
     if (!Ccol || !Cval || !Crp || (batch && !bases))
@@ -847,8 +910,10 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         if (b == BIN_GLOBAL) {
             // the compensated form's split rows need an (s, c) pair per nonzero of theirs
             const unsigned long long scratch_bytes = h2.wide_split_nnz * sizeof(unsigned long long);
-            const unsigned long long scratch_max =
+            unsigned long long scratch_max =
                 opts.wide_scratch_mb > 0 ? (unsigned long long)opts.wide_scratch_mb << 20 : c->total_mem / 4;
+            if (scratch_max > ar.remaining() / 2) // allocation-free: what the workspace has left
+                scratch_max = ar.remaining() / 2;
             const bool split_items = keep_dirs && (!fp32 || scratch_bytes <= scratch_max);
             wide.blocks = split_items || cnt > 2 * c->num_cus ? 2 * c->num_cus : cnt;
             wide.cur_stride = split_items ? 0 : ((long long)h2.num.max_na[b] + 63) & ~63LL;
@@ -1174,7 +1239,7 @@ int run_batch(hsmu_ctx *c, int64_t count, const hsmu_csr *A, const hsmu_csr *B, 
             return HSMU_OK;
         const bool rp64 = opts.rowptr64 != 0;
         const size_t rpb = (size_t)(M + count) * (rp64 ? 8 : 4);
-        void *rp = ar.out(rpb, opts.plain_alloc != 0);
+        void *rp = ar.out(rpb, opts.plain_alloc != 0, OUT_ROWPTR);
         if (!rp)
             return fail(HSMU_ERR_NOMEM, "allocating C row pointers");
         HIP_TRY(launch_fill_zero(rp, rpb, s));
@@ -1258,6 +1323,57 @@ extern "C" int hsmu_spgemm(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hs
     Arena ar(c, nullptr, false);
     bool empty;
     return run_batch(c, 1, A, B, C, opts, ar, RunMode(), stats, &empty);
+}
+
+static_assert(HSMU_NOALLOC_SCRATCH_BYTES == NOALLOC_SCRATCH_BYTES, "scratch size in hsmu.h");
+
+extern "C" int hsmu_spgemm_noalloc(hsmu_ctx *c, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C, int64_t cap_nnz,
+                                   const hsmu_opts *opts_in, hsmu_stats *stats)
+{
+    if (!c || !C || cap_nnz < 0)
+        return fail(HSMU_ERR_INVALID, "hsmu_spgemm_noalloc: ctx or C is NULL, or cap_nnz < 0");
+    const hsmu_opts opts = resolve_opts(opts_in);
+    int rc = check_opts(opts);
+    if (rc)
+        return rc;
+    const bool fp32 = opts.precision == HSMU_FP32_COMP;
+    rc = check_pair(A, B, fp32);
+    if (rc)
+        return rc;
+    if (!C->rowptr || (cap_nnz > 0 && (!C->col || !C->val)))
+        return fail(HSMU_ERR_INVALID, "hsmu_spgemm_noalloc: C's row pointers, or its columns/values with "
+                                      "cap_nnz > 0, are NULL");
+    if (!c->scratch)
+        return fail(HSMU_ERR_UNSUPPORTED, "hsmu_spgemm_noalloc: the scratch array is not available on device %d",
+                    c->device);
+    HIP_TRY(hipSetDevice(c->device));
+    if (stats)
+        memset(stats, 0, sizeof(*stats));
+    // every call on this device shares the array: one at a time in this process
+    std::lock_guard<std::mutex> lk(scratch_lock(c->device));
+    const FixedOut fo{C->rowptr, C->col, C->val, cap_nnz, fp32 ? sizeof(float) : sizeof(double)};
+    Arena ar(c, (char *)c->scratch, NOALLOC_SCRATCH_BYTES, &fo);
+    hsmu_csr R;
+    memset(&R, 0, sizeof(R));
+    bool empty;
+    rc = run_batch(c, 1, A, B, &R, opts, ar, RunMode(), stats, &empty);
+    if (rc == HSMU_ERR_NOMEM && ar.want_nnz() > 0) {
+        C->nnz = ar.want_nnz();
+        return fail(HSMU_ERR_NOMEM, "hsmu_spgemm_noalloc: nnz(C) = %lld exceeds the capacity of C's arrays (%lld)",
+                    (long long)ar.want_nnz(), (long long)cap_nnz);
+    }
+    if (rc == HSMU_ERR_NOMEM && ar.short_bytes() > 0)
+        return fail(HSMU_ERR_NOMEM, "hsmu_spgemm_noalloc: the workspace needs more than the %zu MiB scratch array "
+                                    "(a request reached byte %zu)",
+                    NOALLOC_SCRATCH_BYTES >> 20, ar.short_bytes());
+    if (rc)
+        return rc;
+    C->M = R.M;
+    C->N = R.N;
+    C->nnz = R.nnz;
+    C->rowptr64 = R.rowptr64;
+    C->val32 = R.val32;
+    return HSMU_OK;
 }
 
 extern "C" int hsmu_spgemm_rows(hsmu_ctx *c, const hsmu_csr *A, int64_t r0, int64_t r1, const hsmu_csr *B, hsmu_csr *C,

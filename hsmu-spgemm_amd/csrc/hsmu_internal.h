@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include <hip/hip_runtime_api.h>
 
 namespace hsmu {
@@ -361,6 +363,15 @@ hipError_t launch_plan_bases(const long long *got, const long long *want, int64_
                              hipStream_t s);
 
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cus, hipStream_t s);
+
+// ---- allocation-free entry point (noalloc.hip) ----
+// Bytes of the file-scope device array that is hsmu_spgemm_noalloc's whole workspace.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)256 << 20;
+// Device address of that array on `device`, which must be the current device: looked up
+// once per device (hipGetSymbolAddress), then cached.
+hipError_t scratch_setup(int device, void **ptr);
+// Held for the whole of an hsmu_spgemm_noalloc call: calls on one device share its array.
+std::mutex &scratch_lock(int device);
 
 // Opt every kernel in to the device's full LDS size; call once per device.
 hipError_t configure_kernels(int lds_max);

@@ -201,6 +201,27 @@ const char *hsmu_last_error(void);          /* message of the last failure on th
 int hsmu_spgemm(hsmu_ctx *ctx, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C,
                 const hsmu_opts *opts, hsmu_stats *stats);
 
+/* hsmu_spgemm that allocates nothing: no hipMalloc, hipFree or other allocating call.
+ * No counterpart in the reference (its NHC_spgemm cudaMallocs C and its workspaces
+ * inside the call, src/compressed/compressed_NHC_spgemm.cu:110-113); for callers that
+ * cannot allocate device memory: code under stream capture, serving loops, harnesses
+ * that forbid it.
+ * The caller passes C's arrays: C->rowptr (device, M + 1 entries: int32, or int64 with
+ * opts->rowptr64), C->col (int32) and C->val (double, or float for HSMU_FP32_COMP), each
+ * with room for cap_nnz entries; on success M, N, nnz, rowptr64 and val32 of *C are set.
+ * The workspace is one file-scope device array of HSMU_NOALLOC_SCRATCH_BYTES that every
+ * GPU the library runs on holds until the process ends; a product whose workspace needs
+ * more fails with HSMU_ERR_NOMEM and a message naming the limit (the banded 62,451-row
+ * benchmark product needs about 110 MiB).  If nnz(C) exceeds cap_nnz the call fails with
+ * HSMU_ERR_NOMEM and sets C->nnz to the nnz(C) needed.  Every call on one GPU uses that
+ * GPU's one array, which keeps the last call's contents: calls that overlapped on a GPU
+ * would race, so the library serialises the calls of a process per device (a call waits
+ * for the one before it, whichever thread or context made it).  Synchronous on return,
+ * like hsmu_spgemm; opts->plain_alloc is ignored. */
+#define HSMU_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int hsmu_spgemm_noalloc(hsmu_ctx *ctx, const hsmu_csr *A, const hsmu_csr *B, hsmu_csr *C, int64_t cap_nnz,
+                        const hsmu_opts *opts, hsmu_stats *stats);
+
 /* Release C's device arrays (pool or hipFree, as allocated) and zero the struct. */
 int hsmu_csr_free(hsmu_ctx *ctx, hsmu_csr *C);
 
