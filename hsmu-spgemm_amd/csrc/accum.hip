@@ -33,6 +33,10 @@
 
 namespace hsmu {
 
+// XCDs of the device the row kernels map their row groups onto (configure_kernels sets
+// it per device; each device holds its own copy).
+static __constant__ int c_nxcd = MAX_XCDS;
+
 // Stage the row's A entries a0 .. a0+nA-1 into LDS: first tile and tile count of B_j,
 // and (numeric) first element and length of B_j and a_ij.  One parallel step per row,
 // so the dependent loads Acol -> tptr/Brp are paid once per row rather than once per
@@ -357,7 +361,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_symbolic_lds(const in
     const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
     unsigned char *slice = lds + (size_t)g * slice_bytes;
     const int nrows = *count_p;
-    for (GroupWalk gw(queue, (nrows + RPB - 1) / RPB, grp_s); gw.more(); gw.next()) {
+    for (GroupWalk gw(queue, (nrows + RPB - 1) / RPB, grp_s, c_nxcd); gw.more(); gw.next()) {
         const int idx = gw.grp * RPB + g;
         const int row = idx < nrows ? rows[idx] : -1;
         int S = 0, mn = 0, a0 = 0, nA = 0, tp = 0;
@@ -605,7 +609,7 @@ __global__ __launch_bounds__(G < BLOCK ? BLOCK : G) void k_numeric_lds(const int
     const int g = threadIdx.x / G, l = threadIdx.x & (G - 1);
     unsigned char *slice = lds + (size_t)g * slice_bytes;
     const int nrows = *count_p;
-    for (GroupWalk gw(queue, (nrows + RPB - 1) / RPB, grp_s); gw.more(); gw.next()) {
+    for (GroupWalk gw(queue, (nrows + RPB - 1) / RPB, grp_s, c_nxcd); gw.more(); gw.next()) {
         const int idx = gw.grp * RPB + g;
         const int row = idx < nrows ? rows[idx] : -1;
         int S = 0, mn = 0, n = 0, a0 = 0, nA = 0, tp = 0;
@@ -1878,8 +1882,6 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
 {
     if (bin == BIN_GLOBAL) {
         const int blocks = count < wide.blocks ? count : wide.blocks;
-        *lds_bytes += op.dir_mask && wide.items ? (unsigned long long)wide.blocks * WS_LDS
-                                                : (unsigned long long)blocks * (op.dir_mask ? CW_LDS : WIN_LDS);
         if (op.dir_mask && wide.items) { // split work items (FP64, or compensated with its scratch)
             constexpr bool F32C = std::is_same<P, PrecF32C>::value;
             hipError_t e = launch_fill_zero(wide.ctr, WIDE_CTRS * sizeof(unsigned), s);
@@ -1946,9 +1948,10 @@ size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks)
     return (size_t)blocks * (size_t)(3 * cur_stride + ft_stride);
 }
 
-hipError_t configure_kernels(int lds_max)
+hipError_t configure_kernels(int lds_max, int num_xcds)
 {
-    hipError_t e = hipSuccess;
+    const int nx = num_xcds < 1 ? 1 : (num_xcds > MAX_XCDS ? MAX_XCDS : num_xcds);
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_nxcd), &nx, sizeof(int));
 #define CFG(K)                                                                                                         \
     {                                                                                                                  \
         hipError_t r = hipFuncSetAttribute((const void *)(K), hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);    \

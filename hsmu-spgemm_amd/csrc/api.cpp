@@ -180,6 +180,7 @@ struct hsmu_ctx {
     int num_cus = 0;
     int lds_max = 0;
     size_t total_mem = 0; // device memory (bytes)
+    int num_xcds = 0;     // XCDs (hipDeviceAttributeNumberOfXccs)
     void *scratch = nullptr; // device address of hsmu_spgemm_noalloc's array on this device
 };
 
@@ -229,6 +230,11 @@ int hsmu_ctx_create(int device, hsmu_ctx **out)
     c->num_cus = prop.multiProcessorCount;
     c->lds_max = (int)prop.sharedMemPerBlock;
     c->total_mem = prop.totalGlobalMem;
+    if (hipDeviceGetAttribute(&c->num_xcds, hipDeviceAttributeNumberOfXccs, device) != hipSuccess ||
+        c->num_xcds < 1) {
+        (void)hipGetLastError();
+        c->num_xcds = MAX_XCDS;
+    }
     // every failure from here on releases what was already created
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
               hipHostMalloc((void **)&c->host_stats, sizeof(DevStats), hipHostMallocDefault) == hipSuccess;
@@ -259,7 +265,7 @@ int hsmu_ctx_create(int device, hsmu_ctx **out)
             // Best effort: HIP on AMD accepts dynamic LDS up to the device limit without
             // the attribute; a refusal here is not an error, a launch that needs more
             // LDS than allowed reports its own.
-            if (configure_kernels(c->lds_max - LDS_STATIC_RESERVE) != hipSuccess)
+            if (configure_kernels(c->lds_max - LDS_STATIC_RESERVE, c->num_xcds) != hipSuccess)
                 (void)hipGetLastError();
             configured.push_back(device);
         }
@@ -1017,8 +1023,11 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         S.compulsory_bytes = (4.0 * (double)(M + 1) + (4.0 + vsz) * (double)P.nnzA) +
                              (self ? 0.0 : 4.0 * (double)(K + 1) + (4.0 + vsz) * (double)P.nnzB) +
                              (rp64 ? 8.0 : 4.0) * (double)(M + 1) + (4.0 + vsz) * (double)nnzc;
+        unsigned long long nnz_lds = 0; // nonzeros of the rows in LDS bins
+        for (int b = 0; b < NLDS; ++b)
+            nnz_lds += h2.num.nnz_c[b];
         S.lds_launched = (double)lds_launched;
-        S.lds_use = lds_launched ? 12.0 * (double)nnzc / (double)lds_launched : 0.0;
+        S.lds_use = lds_launched ? 12.0 * (double)nnz_lds / (double)lds_launched : 0.0;
     }
     return HSMU_OK;
 }

@@ -63,29 +63,29 @@ template <typename T> DEVI T wave_incl_scan(T v)
 }
 
 // Map the hardware block index so that each XCD (blocks are dealt round-robin over the
-// 8 XCDs) works on a contiguous range of logical blocks: neighbouring rows, which share
-// rows of B, then share one XCD's L2.
-DEVI int xcd_block(int b, int n)
+// nx XCDs of the device: 8 on an MI355X) works on a contiguous range of logical blocks:
+// neighbouring rows, which share rows of B, then share one XCD's L2.
+DEVI int xcd_block(int b, int n, int nx)
 {
-    const int x = b & 7, q = b >> 3, per = n >> 3, rem = n & 7;
+    const int x = b % nx, q = b / nx, per = n / nx, rem = n % nx;
     return x * per + (x < rem ? x : rem) + q;
 }
 
 // Device work queue of the persistent row kernels: the n row groups of a launch are cut
-// into 8 contiguous ranges, one per XCD, each with a counter q[x] (zero at launch) that
+// into nx contiguous ranges, one per XCD, each with a counter q[x] (zero at launch) that
 // hands out chunks of `chunk` consecutive groups.  A workgroup takes the next chunk of its
-// own XCD's range (xq starts at blockIdx.x & 7) and, once that range is used up, of the
+// own XCD's range (xq starts at blockIdx.x % nx) and, once that range is used up, of the
 // following ranges in turn, so the groups one XCD works on stay neighbours (shared rows
 // of B in its L2) until the queues drain.  Thread 0 pulls; the chunk [s[0], s[1]) goes to
 // every thread through LDS, s[0] = n when nothing is left.  The caller's loop body must
 // end with a workgroup barrier before the next pull.
-DEVI void pull_chunk(unsigned *q, int n, int chunk, int &xq, int *s)
+DEVI void pull_chunk(unsigned *q, int n, int chunk, int nx, int &xq, int *s)
 {
     if (threadIdx.x == 0) {
         int got = n, end = n;
-        for (int k = 0; k < 8; ++k) {
-            const int x = (xq + k) & 7;
-            const int lo = (int)(((long long)n * x) >> 3), hi = (int)(((long long)n * (x + 1)) >> 3);
+        for (int k = 0; k < nx; ++k) {
+            const int x = (xq + k) % nx;
+            const int lo = (int)((long long)n * x / nx), hi = (int)((long long)n * (x + 1) / nx);
             const unsigned nch = (unsigned)((hi - lo + chunk - 1) / chunk);
             if (__hip_atomic_load(&q[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= nch)
                 continue;
@@ -105,22 +105,24 @@ DEVI void pull_chunk(unsigned *q, int n, int chunk, int &xq, int *s)
 
 // The row groups a workgroup of a row kernel works on: with a queue, chunks pulled from
 // it; without, groups xcd_block(blockIdx) + k * gridDim (one group at a time).
+// nx: XCDs of the device (1..MAX_XCDS).
 struct GroupWalk {
     unsigned *q;
-    int n, chunk, xq, grp, end;
+    int n, chunk, nx, xq, grp, end;
     int *s; // two ints of LDS
 
-    DEVI GroupWalk(unsigned *queue, int ngroups, int *lds2) : q(queue), n(ngroups), xq(blockIdx.x & 7), s(lds2)
+    DEVI GroupWalk(unsigned *queue, int ngroups, int *lds2, int nxcd)
+        : q(queue), n(ngroups), nx(nxcd), xq(blockIdx.x % nxcd), s(lds2)
     {
         // chunks of about an eighth of a workgroup's share of its XCD's range
         const long long per = (long long)ngroups / ((long long)gridDim.x * 8);
         chunk = per > 1 ? (int)per : 1;
         if (q) {
-            pull_chunk(q, n, chunk, xq, s);
+            pull_chunk(q, n, chunk, nx, xq, s);
             grp = s[0];
             end = s[1];
         } else {
-            grp = xcd_block(blockIdx.x, gridDim.x);
+            grp = xcd_block(blockIdx.x, gridDim.x, nx);
             end = grp + 1;
         }
     }
@@ -130,7 +132,7 @@ struct GroupWalk {
         if (++grp < end)
             return;
         if (q) {
-            pull_chunk(q, n, chunk, xq, s);
+            pull_chunk(q, n, chunk, nx, xq, s);
             grp = s[0];
             end = s[1];
         } else {

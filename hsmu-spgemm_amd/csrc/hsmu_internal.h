@@ -51,6 +51,11 @@ struct BinStats {
     int max_na[NBINS];              // longest row of A in the bin
 };
 
+// XCDs (accelerator complex dies, each with its own L2) the kernels map work onto: the
+// device's count (hipDeviceAttributeNumberOfXccs; 8 on an MI355X, fewer in partitioned
+// modes), at most this many.
+constexpr int MAX_XCDS = 8;
+
 // Device-side counters of one call, copied to pinned host memory at the two
 // synchronisation points.
 struct DevStats {
@@ -65,7 +70,7 @@ struct DevStats {
     BinStats sym;
     BinStats num;
     // work-queue counters of the persistent LDS kernels: [phase][bin][XCD]
-    unsigned queue[2][NBINS][8];
+    unsigned queue[2][NBINS][MAX_XCDS];
 };
 
 // Batched form: instance i of a batch, as seen by the stacking kernels.  The stacked A
@@ -318,8 +323,10 @@ struct WideRows {
 size_t wide_ws_ints(long long cur_stride, long long ft_stride, int blocks);
 
 // Numeric phase of one bin.  fp32c = 0: FP64 values (Cval double); fp32c = 1: compensated
-// FP32 (Cval float).  lds_bytes (may be null): += the dynamic LDS of the launch, workgroups
-// x bytes per workgroup (the reference's "practical need", compressed_Cval.cuh:202-533).
+// FP32 (Cval float).  lds_bytes (may be null): += the dynamic LDS of an LDS-bin launch,
+// workgroups x bytes per workgroup (the reference's "practical need",
+// compressed_Cval.cuh:202-533); the wide-row kernels' persistent workgroups reuse their LDS
+// window by window and are not counted.
 hipError_t launch_numeric(int bin, int fp32c, const int *rows, const int *count_dev, int count,
                           int max_need, const Operands &op, const long long *crp, int *Ccol,
                           void *Cval, int bshift, int tshift, int two_level, int long_rows,
@@ -373,7 +380,8 @@ hipError_t scratch_setup(int device, void **ptr);
 // Held for the whole of an hsmu_spgemm_noalloc call: calls on one device share its array.
 std::mutex &scratch_lock(int device);
 
-// Opt every kernel in to the device's full LDS size; call once per device.
-hipError_t configure_kernels(int lds_max);
+// Opt every kernel in to the device's full LDS size and tell the row kernels the device's
+// XCD count (1..MAX_XCDS); call once per device, with it current.
+hipError_t configure_kernels(int lds_max, int num_xcds);
 
 } // namespace hsmu

@@ -163,10 +163,11 @@ typedef struct hsmu_stats {
     int64_t num_bin_products[HSMU_NBINS];
     double compulsory_bytes;  /* every input array read once, C written once: the lower bound
                                  of HBM traffic */
-    double lds_launched;      /* dynamic LDS of the numeric launches, workgroups x bytes each
-                                 (the reference's practical_need, compressed_Cval.cuh:202-533) */
-    double lds_use;           /* nnz(C) * 12 / lds_launched: the reference's use_of_shared
-                                 (compressed_Cval.cuh:564-570) */
+    double lds_launched;      /* dynamic LDS of the numeric LDS-bin launches, workgroups x
+                                 bytes each (the reference's practical_need,
+                                 compressed_Cval.cuh:202-533) */
+    double lds_use;           /* 12 * nnz(C) of the LDS-bin rows / lds_launched: the
+                                 reference's use_of_shared (compressed_Cval.cuh:564-570) */
 } hsmu_stats;
 
 /* ---- context ------------------------------------------------------------------- */
