@@ -231,7 +231,7 @@ def timed_calls(ctx, dA, dB, opts, steps, warmup, r0=None, r1=None):
     return statistics.median(ms), st
 
 
-def bench_rmat(hsmu, ctx, steps):
+def bench_rmat(hsmu, ctx, steps, table):
     """R-MAT scale 20 (a, b, c = .57, .19, .19, edge factor 16, duplicates merged, values
     U(0,1), seed 2), C = A*A with int64 row pointers, FP64 and compensated FP32 (values
     rounded to FP32): median of `steps` calls after one warm-up, the wide-row bin's time
@@ -243,8 +243,9 @@ def bench_rmat(hsmu, ctx, steps):
     for name, fp32 in (("fp64", False), ("fp32_compensated", True)):
         Am = as_fp32(hsmu, A) if fp32 else A
         dA = ctx.matrix(Am, val32=fp32)
-        opts = hsmu.default_opts(rowptr64=1, timing=2,
-                                 precision=hsmu.HSMU_FP32_COMP if fp32 else hsmu.HSMU_FP64)
+        base = hsmu.default_opts(rowptr64=1, timing=2, precision=hsmu.HSMU_FP32_COMP if fp32 else hsmu.HSMU_FP64)
+        opts, found = ctx.autotune([dA], [dA], base=base, table=table if os.path.exists(table) else None,
+                                   search=False)
         ms, st = timed_calls(ctx, dA, dA, opts, steps, 1)
         wide = 8
         r = {"ms": round(ms, 2), "gflops": round(2.0 * st["products"] / (ms * 1e6), 2),
@@ -258,7 +259,9 @@ def bench_rmat(hsmu, ctx, steps):
                           "alg_bytes": st["num_bin_bytes"][wide],
                           "alg_GBps": round(st["num_bin_bytes"][wide] / (st["ms_numeric_bin"][wide] * 1e-3) / 1e9, 1)
                           if st["ms_numeric_bin"][wide] > 0 else None},
-             "ms_numeric_bins": [round(x, 2) for x in st["ms_numeric_bin"]], "lds_use": round(st["lds_use"], 4)}
+             "ms_numeric_bins": [round(x, 2) for x in st["ms_numeric_bin"]], "lds_use": round(st["lds_use"], 4),
+             "shape_class": hsmu.shape_class([dA], [dA]), "tuned_from_table": bool(found),
+             "opts": {k: getattr(opts, k) for k in OPT_KEYS}}
         c, _ = ctx.spgemm_device(dA, dA, opts)
         try:
             r["errors"] = sampled_row_errors(hsmu, ctx, c, Am, Am, sample_rows(ctx, c, 16, 32, 20), 0, fp32)
@@ -272,7 +275,14 @@ def bench_rmat(hsmu, ctx, steps):
     return out
 
 
-def bench_zipf(hsmu, ctx, panel_rows=10000):
+class _Shape:
+    """Sizes of a row panel as the autotuner's shape class sees them (no arrays)."""
+
+    def __init__(self, hsmu, M, N, nnz):
+        self.csr = hsmu.Csr(M, N, nnz, 0, 0, 0, 0, 0)
+
+
+def bench_zipf(hsmu, ctx, table, panel_rows=10000):
     """Zipf A*A^T, A 500,000 x 100,000, 8 distinct columns per row Zipf(1.2), values
     U(-1,1), seed 4: C (about 2.7 TB) streamed in row panels (hsmu_spgemm_rows), every
     product of the full A*A^T computed, in FP64 and compensated FP32; device time summed
@@ -286,7 +296,11 @@ def bench_zipf(hsmu, ctx, panel_rows=10000):
         Am = as_fp32(hsmu, A) if fp32 else A
         Bm = hsmu.transpose(Am)
         dA, dB = ctx.matrix(Am, val32=fp32), ctx.matrix(Bm, val32=fp32)
-        opts = hsmu.default_opts(rowptr64=1, timing=1, precision=hsmu.HSMU_FP32_COMP if fp32 else hsmu.HSMU_FP64)
+        base = hsmu.default_opts(rowptr64=1, timing=1, precision=hsmu.HSMU_FP32_COMP if fp32 else hsmu.HSMU_FP64)
+        r0 = 250000  # the tuned panel's class (every full panel has about the same sizes)
+        pa = [_Shape(hsmu, panel_rows, A.N, int(A.rowptr[r0 + panel_rows] - A.rowptr[r0]))]
+        opts, found = ctx.autotune(pa, [dB], base=base, table=table if os.path.exists(table) else None,
+                                   search=False)
         c, _ = ctx.spgemm_rows_device(dA, 0, panel_rows, dB, opts)  # untimed: the pool's first allocations
         ctx.free_result(c)
         products = nnz_c = 0
@@ -305,8 +319,9 @@ def bench_zipf(hsmu, ctx, panel_rows=10000):
              "ms_device": round(dev, 1), "ms_wall": round(wall, 1),
              "gflops_device": round(2.0 * products / (dev * 1e6), 2),
              "gflops_wall": round(2.0 * products / (wall * 1e6), 2),
-             "frac_end_to_end_device": round(alg / (dev * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
-        r0 = 250000
+             "frac_end_to_end_device": round(alg / (dev * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+             "shape_class": hsmu.shape_class(pa, [dB]), "tuned_from_table": bool(found),
+             "opts": {k: getattr(opts, k) for k in OPT_KEYS}}
         c, _ = ctx.spgemm_rows_device(dA, r0, r0 + panel_rows, dB, opts)
         try:
             r["errors"] = sampled_row_errors(hsmu, ctx, c, Am, Bm, sample_rows(ctx, c, 8, 56, r0), r0, fp32)
@@ -483,10 +498,10 @@ def main():
     ctx.trim()
     rmat = None
     if not args.no_rmat:
-        rmat = bench_rmat(hsmu, ctx, 3)
+        rmat = bench_rmat(hsmu, ctx, 3, args.table)
     zipf = None
     if not args.no_zipf:
-        zipf = bench_zipf(hsmu, ctx)
+        zipf = bench_zipf(hsmu, ctx, args.table)
 
     cpu = None
     if not args.no_cpu:

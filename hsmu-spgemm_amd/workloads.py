@@ -2,7 +2,7 @@
 
     python hsmu-spgemm_amd/workloads.py rmat [--scale 20] [--steps 3] [--fp32c]
     python hsmu-spgemm_amd/workloads.py zipf [--rows 500000] [--cols 100000] [--panel-rows 10000] [--fp32c]
-    python hsmu-spgemm_amd/workloads.py batch [--replays 10]
+    python hsmu-spgemm_amd/workloads.py batch [--replays 10] [--subset all|small|large]
 
 rmat: C = A*A, A from R-MAT (a, b, c = .57, .19, .19), edge factor 16, duplicates merged,
 values U(0,1), seed 2; int64 row pointers (nnz(C) passes 2^31 at scale 20).
@@ -12,7 +12,8 @@ in device memory, so it is computed in row panels of A (hsmu_spgemm_rows), each 
 released before the next: the run covers every product of the full A*A^T.
 batch: the 512 uneven Laplacian products (256 2D with side floor(e^U(ln16,ln256)), 256
 3D with side floor(e^U(ln8,ln48)), seed 3) as a plan with the autotuner table's options,
-replayed as one HIP graph.
+replayed as one HIP graph, beside the same products as sequential calls (timed in C);
+--subset small keeps the instances with N < 4096, --subset large the others.
 
 Check (rmat, zipf): every row sum of C equals A times the row sums of B (hsmu_csr_row_sums
 on the device, the right side with scipy on the host), within 1e-12 of the row's sum of
@@ -149,7 +150,15 @@ def run_batch(args):
         raise SystemExit("--replays must be at least 1")
     ctx = hsmu.Context(0)
     mats = hsmu.gen_laplace_batch(512, seed=3)
+    # subsets: the instances below the reference's small-path threshold (N < 4096, which
+    # src/NHC_spgemm.cuh:7-43 sends to the dense-bitmap path), or the others
+    if args.subset == "small":
+        mats = [m for m in mats if m.N < 4096]
+    elif args.subset == "large":
+        mats = [m for m in mats if m.N >= 4096]
     dms = [ctx.matrix(m) for m in mats]
+    ctx.spgemm_each_ms(dms, dms, hsmu.default_opts(timing=0), 1)  # warm-up: the pool's first allocations
+    ms_seq = ctx.spgemm_each_ms(dms, dms, hsmu.default_opts(timing=0), 3)
     table = os.path.join(HERE, "tuning", "gfx950.tsv")
     opts, found = ctx.autotune(dms, dms, base=hsmu.default_opts(timing=0), table=table if os.path.exists(table) else None,
                                search=False)
@@ -159,9 +168,10 @@ def run_batch(args):
         st = plan.execute()
         dev.append(st["ms_total"])
     out = {"workload": "512 uneven Laplacian products (256 2D, 256 3D, seed 3) as one plan, replayed as a HIP graph",
-           "instances": len(mats), "rows": sum(m.M for m in mats), "replays": args.replays,
+           "subset": args.subset, "instances": len(mats), "rows": sum(m.M for m in mats), "replays": args.replays,
            "ms_graph_device_median": round(float(np.median(dev)), 3), "products": st["products"],
-           "nnz_c": st["nnz_c"], "tuned_from_table": bool(found)}
+           "nnz_c": st["nnz_c"], "tuned_from_table": bool(found),
+           "ms_sequential_calls": round(ms_seq, 3)}
     print(json.dumps(out), flush=True)
     plan.destroy()
     for d in dms:
@@ -183,6 +193,8 @@ def main():
     z.add_argument("--fp32c", action="store_true")
     b = sub.add_parser("batch")
     b.add_argument("--replays", type=int, default=10, help="graph replays after the plan's recording run (at least 1)")
+    b.add_argument("--subset", choices=("all", "small", "large"), default="all",
+                   help="all 512 instances, only those with N < 4096, or only the others")
     args = ap.parse_args()
     if args.what == "rmat":
         run_rmat(args)
