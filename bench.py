@@ -297,8 +297,8 @@ def bench_zipf(hsmu, ctx, table, panel_rows=10000):
         Bm = hsmu.transpose(Am)
         dA, dB = ctx.matrix(Am, val32=fp32), ctx.matrix(Bm, val32=fp32)
         base = hsmu.default_opts(rowptr64=1, timing=1, precision=hsmu.HSMU_FP32_COMP if fp32 else hsmu.HSMU_FP64)
-        r0 = 250000  # the tuned panel's class (every full panel has about the same sizes)
-        pa = [_Shape(hsmu, panel_rows, A.N, int(A.rowptr[r0 + panel_rows] - A.rowptr[r0]))]
+        e0 = 250000  # the panel whose rows are checked; its class is every full panel's
+        pa = [_Shape(hsmu, panel_rows, A.N, int(A.rowptr[e0 + panel_rows] - A.rowptr[e0]))]
         opts, found = ctx.autotune(pa, [dB], base=base, table=table if os.path.exists(table) else None,
                                    search=False)
         c, _ = ctx.spgemm_rows_device(dA, 0, panel_rows, dB, opts)  # untimed: the pool's first allocations
@@ -322,10 +322,10 @@ def bench_zipf(hsmu, ctx, table, panel_rows=10000):
              "frac_end_to_end_device": round(alg / (dev * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
              "shape_class": hsmu.shape_class(pa, [dB]), "tuned_from_table": bool(found),
              "opts": {k: getattr(opts, k) for k in OPT_KEYS}}
-        c, _ = ctx.spgemm_rows_device(dA, r0, r0 + panel_rows, dB, opts)
+        c, _ = ctx.spgemm_rows_device(dA, e0, e0 + panel_rows, dB, opts)
         try:
-            r["errors"] = sampled_row_errors(hsmu, ctx, c, Am, Bm, sample_rows(ctx, c, 8, 56, r0), r0, fp32)
-            r["errors"]["rows_sampled_from"] = f"panel [{r0}, {r0 + panel_rows}): 8 widest + 56 random rows"
+            r["errors"] = sampled_row_errors(hsmu, ctx, c, Am, Bm, sample_rows(ctx, c, 8, 56, e0), e0, fp32)
+            r["errors"]["rows_sampled_from"] = f"panel [{e0}, {e0 + panel_rows}): 8 widest + 56 random rows"
         finally:
             ctx.free_result(c)
         dA.free()

@@ -148,7 +148,7 @@ for _name, (_res, _args) in _sig.items():
 
 
 def build_info() -> str:
-    """The loaded library's stamp: "head=<commit>[+dirty] src=<hash>"."""
+    """The loaded library's stamp: "src_commit=<commit>[+dirty] src=<hash>"."""
     return lib.hsmu_build_info().decode()
 
 

@@ -2,7 +2,7 @@
 
     python hsmu-spgemm_amd/workloads.py rmat [--scale 20] [--steps 3] [--fp32c]
     python hsmu-spgemm_amd/workloads.py zipf [--rows 500000] [--cols 100000] [--panel-rows 10000] [--fp32c]
-    python hsmu-spgemm_amd/workloads.py batch [--replays 10] [--subset all|small|large]
+    python hsmu-spgemm_amd/workloads.py batch [--replays 10] [--subset all|small|large] [--no-sequential]
 
 rmat: C = A*A, A from R-MAT (a, b, c = .57, .19, .19), edge factor 16, duplicates merged,
 values U(0,1), seed 2; int64 row pointers (nnz(C) passes 2^31 at scale 20).
@@ -73,6 +73,9 @@ def run_rmat(args):
     dA = ctx.matrix(A, val32=args.fp32c)
     opts = hsmu.default_opts(rowptr64=1, timing=1,
                              precision=hsmu.HSMU_FP32_COMP if args.fp32c else hsmu.HSMU_FP64)
+    if args.tuned:
+        table = os.path.join(HERE, "tuning", "gfx950.tsv")
+        opts, _ = ctx.autotune([dA], [dA], base=opts, table=table, search=False)
     ms = []
     for it in range(args.steps + 1):
         t1 = time.perf_counter()
@@ -157,8 +160,10 @@ def run_batch(args):
     elif args.subset == "large":
         mats = [m for m in mats if m.N >= 4096]
     dms = [ctx.matrix(m) for m in mats]
-    ctx.spgemm_each_ms(dms, dms, hsmu.default_opts(timing=0), 1)  # warm-up: the pool's first allocations
-    ms_seq = ctx.spgemm_each_ms(dms, dms, hsmu.default_opts(timing=0), 3)
+    ms_seq = None
+    if not args.no_sequential:
+        ctx.spgemm_each_ms(dms, dms, hsmu.default_opts(timing=0), 1)  # warm-up: the pool's first allocations
+        ms_seq = round(ctx.spgemm_each_ms(dms, dms, hsmu.default_opts(timing=0), 3), 3)
     table = os.path.join(HERE, "tuning", "gfx950.tsv")
     opts, found = ctx.autotune(dms, dms, base=hsmu.default_opts(timing=0), table=table if os.path.exists(table) else None,
                                search=False)
@@ -171,7 +176,7 @@ def run_batch(args):
            "subset": args.subset, "instances": len(mats), "rows": sum(m.M for m in mats), "replays": args.replays,
            "ms_graph_device_median": round(float(np.median(dev)), 3), "products": st["products"],
            "nnz_c": st["nnz_c"], "tuned_from_table": bool(found),
-           "ms_sequential_calls": round(ms_seq, 3)}
+           "ms_sequential_calls": ms_seq}
     print(json.dumps(out), flush=True)
     plan.destroy()
     for d in dms:
@@ -186,6 +191,7 @@ def main():
     r.add_argument("--scale", type=int, default=20)
     r.add_argument("--steps", type=int, default=3)
     r.add_argument("--fp32c", action="store_true")
+    r.add_argument("--tuned", action="store_true", help="options from the autotuner table for the class")
     z = sub.add_parser("zipf")
     z.add_argument("--rows", type=int, default=500000)
     z.add_argument("--cols", type=int, default=100000)
@@ -195,6 +201,8 @@ def main():
     b.add_argument("--replays", type=int, default=10, help="graph replays after the plan's recording run (at least 1)")
     b.add_argument("--subset", choices=("all", "small", "large"), default="all",
                    help="all 512 instances, only those with N < 4096, or only the others")
+    b.add_argument("--no-sequential", action="store_true",
+                   help="skip the sequential calls (profiles of the plan's kernels alone)")
     args = ap.parse_args()
     if args.what == "rmat":
         run_rmat(args)

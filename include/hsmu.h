@@ -189,8 +189,9 @@ int hsmu_ctx_num_cus(hsmu_ctx *ctx);
 
 void hsmu_opts_default(hsmu_opts *opts);
 const char *hsmu_status_string(int status);
-/* "head=<commit>[+dirty] src=<hash>": the commit and the source hash this library was
- * built from (no counterpart in the reference). */
+/* "src_commit=<commit>[+dirty] src=<hash>": the last commit that changed the library's
+ * sources and the hash of the sources it was built from (no counterpart in the
+ * reference). */
 const char *hsmu_build_info(void);
 const char *hsmu_last_error(void);          /* message of the last failure on this thread */
 
