@@ -230,7 +230,10 @@ DEVI void chunk_offsets(const u64 *m, const ChunkScan &c, Out out)
 // they cannot feed TwoSum.)
 struct PrecF64 {
     typedef double V;
-    static DEVI void add(double *vals, unsigned slot, double a, V b) { atomicAdd(&vals[slot], a * b); }
+    typedef double T; // a product ready to be added
+    static DEVI T prep(double a, V b) { return a * b; }
+    static DEVI void put(double *vals, unsigned slot, T p) { atomicAdd(&vals[slot], p); }
+    static DEVI void add(double *vals, unsigned slot, double a, V b) { put(vals, slot, prep(a, b)); }
     static DEVI V result(const double *vals, unsigned slot) { return vals[slot]; }
 };
 
@@ -241,11 +244,19 @@ struct PrecF32C {
     // (TwoSum, Knuth) and adds it and e into c; the swap succeeds only if no other lane
     // changed the pair meanwhile, so the error always describes the addition stored.
     // c's own rounding is second order (|c| ~ u |s|).
-    static DEVI void add(double *vals, unsigned slot, double a, V b)
+    struct T { // the product split exactly: p + e = a * b
+        float p, e;
+    };
+    static DEVI T prep(double a, V b)
     {
         const float af = (float)a;
         const float p = __fmul_rn(af, b);
-        const float e = __builtin_fmaf(af, b, -p);
+        return T{p, __builtin_fmaf(af, b, -p)};
+    }
+    static DEVI void add(double *vals, unsigned slot, double a, V b) { put(vals, slot, prep(a, b)); }
+    static DEVI void put(double *vals, unsigned slot, T pe)
+    {
+        const float p = pe.p, e = pe.e;
         unsigned long long *w = (unsigned long long *)vals + slot;
         unsigned long long old = *w;
         while (true) {
@@ -298,19 +309,42 @@ DEVI void add_walk(const Operands &op, int kb, int ke, int gw, int gl, double a,
                 v[u] = Bval[ku];
             }
         }
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u) {
-            if (c[u] >= 0) {
-                unsigned slot;
-                if constexpr (FORM == SLOT_DENSE) {
-                    slot = (unsigned)(c[u] - (mn << 6));
-                } else {
-                    const int t = (c[u] >> 6) - mn;
-                    const unsigned kk = FORM == SLOT_TWO_LEVEL ? fine_index(cm, coff, t) : (unsigned)t;
-                    slot = foff[kk] + (unsigned)__popcll(fm[kk] & below(c[u] & 63));
-                }
-                P::add(vals, slot, a, v[u]);
+        auto slot_of = [&](int col) -> unsigned {
+            if constexpr (FORM == SLOT_DENSE) {
+                return (unsigned)(col - (mn << 6));
+            } else {
+                const int t = (col >> 6) - mn;
+                const unsigned kk = FORM == SLOT_TWO_LEVEL ? fine_index(cm, coff, t) : (unsigned)t;
+                return foff[kk] + (unsigned)__popcll(fm[kk] & below(col & 63));
             }
+        };
+        if constexpr (std::is_same<P, PrecF64>::value) {
+            // every slot and product of the step first, then the adds: a product computed
+            // between two adds reuses the registers the add before it still reads, so each
+            // add waited for the previous one to leave the LDS queue (measured on banded:
+            // 2.5% off the step, 3% off the dominant kernel)
+            unsigned slot[UNROLL];
+            double pv[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                slot[u] = 0;
+                pv[u] = 0.0;
+                if (c[u] >= 0) {
+                    slot[u] = slot_of(c[u]);
+                    pv[u] = P::prep(a, v[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+                if (c[u] >= 0)
+                    P::put(vals, slot[u], pv[u]);
+        } else {
+            // compensated FP32: each add is a compare-and-swap loop that waits for its own
+            // result anyway; separating the steps measured 1.5% slower
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u)
+                if (c[u] >= 0)
+                    P::add(vals, slot_of(c[u]), a, v[u]);
         }
     }
 }
