@@ -1316,9 +1316,15 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
                                                      int *__restrict__ Ccol, double *__restrict__ Cval,
                                                      long long sym_row_max, int sym_all_global, int long_rows)
 {
-    __shared__ unsigned dk[WC_DENSE]; // dense tiles: fine entry
-    __shared__ int dtile[WC_DENSE];   // and tile (absolute)
-    __shared__ int stage[WC_STAGE];
+    // one buffer, used by one path per chunk: the staged chunk's columns, or the list of
+    // its dense tiles (mask, first slot, absolute tile), so that the waves writing them
+    // read LDS instead of going back to the directory in global memory
+    static_assert(WC_DENSE * (8 + 4 + 4) <= WC_STAGE * 4, "dense-tile list fits the stage");
+    __shared__ __align__(16) unsigned char buf[WC_STAGE * 4];
+    int *stage = (int *)buf;
+    u64 *dmask = (u64 *)buf;
+    unsigned *doff = (unsigned *)(buf + 8 * WC_DENSE);
+    int *dtile = (int *)(buf + 12 * WC_DENSE);
     __shared__ unsigned red[WC_BS / 64];
     __shared__ int nd_s;
     const int l = threadIdx.x, lane = l & 63, w = l >> 6;
@@ -1344,29 +1350,37 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
         const unsigned *foff = op.dir_off + rd + W;
         auto emit = [&](int k, int tile) {
             u64 m = fm[k];
+            const unsigned o = foff[k];
             if (__popcll(m) > 4) {
                 const int p = atomicAdd(&nd_s, 1);
                 if (p < WC_DENSE) {
-                    dk[p] = (unsigned)k;
+                    dmask[p] = m;
+                    doff[p] = o;
                     dtile[p] = tile;
                     return;
                 }
             }
-            long long at = cb + foff[k];
+            long long at = cb + o;
             while (m) {
                 Ccol[at++] = (tile << 6) + __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
             }
         };
         const int U = two ? W : S; // units: coarse words, or tiles
+        // flat rows: each chunk's tile masks and first slot are loaded one chunk ahead, so
+        // the load overlaps the previous chunk's scan and stores
+        u64 m_next = !two && l < S ? fm[l] : 0ull;
+        unsigned base_next = !two && S > 0 ? uniform(foff[0]) : 0u;
         for (int u0 = 0; u0 < U; u0 += WC_BS) {
             const int u = u0 + l;
             if (!two) { // staged: [foff[u0], foff[u0] + tot) holds this chunk's columns
-                const u64 m = u < S ? fm[u] : 0ull;
+                const u64 m = m_next;
+                const unsigned base = base_next;
+                m_next = u + WC_BS < S ? fm[u + WC_BS] : 0ull;
+                base_next = u0 + WC_BS < S ? uniform(foff[u0 + WC_BS]) : 0u;
                 unsigned tot;
                 const unsigned pos = group_excl_scan<WC_BS>((unsigned)__popcll(m), red, tot);
                 if (uniform(tot) <= (unsigned)WC_STAGE) {
-                    const unsigned base = uniform(foff[u0]);
                     const int c0 = (mn + u) << 6;
                     u64 mm = m;
                     unsigned q = pos;
@@ -1399,12 +1413,11 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
             __syncthreads();
             const int nd = min(uniform(nd_s), WC_DENSE);
             for (int t = w; t < nd; t += WC_BS / 64) {
-                const int k = (int)dk[t];
-                const u64 m = fm[k];
+                const u64 m = dmask[t];
                 if ((m >> lane) & 1ull)
-                    Ccol[cb + foff[k] + __popcll(m & below(lane))] = (dtile[t] << 6) + lane;
+                    Ccol[cb + doff[t] + __popcll(m & below(lane))] = (dtile[t] << 6) + lane;
             }
-            __syncthreads(); // dk, dtile and nd_s of the next pass
+            __syncthreads(); // the list and nd_s of the next pass
         }
     }
 }
