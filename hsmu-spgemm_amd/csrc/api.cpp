@@ -1366,6 +1366,11 @@ extern "C" int hsmu_spgemm_noalloc(hsmu_ctx *c, const hsmu_csr *A, const hsmu_cs
     memset(&R, 0, sizeof(R));
     bool empty;
     rc = run_batch(c, 1, A, B, &R, opts, ar, RunMode(), stats, &empty);
+    // A call that failed part-way may have left kernels running on this context's stream;
+    // they use the shared array, so drain them before the next call (from any context)
+    // may take it.  (A successful call has waited for its stream already.)
+    if (rc != HSMU_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+        (void)hipGetLastError();
     if (rc == HSMU_ERR_NOMEM && ar.want_nnz() > 0) {
         C->nnz = ar.want_nnz();
         return fail(HSMU_ERR_NOMEM, "hsmu_spgemm_noalloc: nnz(C) = %lld exceeds the capacity of C's arrays (%lld)",
