@@ -1208,13 +1208,13 @@ DEVI int wide_class(long long n)
     return c < 0 ? 0 : (c >= WS_CLASSES ? WS_CLASSES - 1 : c);
 }
 
-DEVI void wide_row_items(const int *rows, int idx, const int *Arp, const long long *crp, int &row, int &nA, int &sz,
-                         int &ns, int &cls)
+DEVI void wide_row_items(const int *rows, int idx, const int *Arp, const long long *crp, int vals, int &row, int &nA,
+                         int &sz, int &ns, int &cls)
 {
     row = rows[idx];
     nA = Arp[row + 1] - Arp[row];
     const long long n = crp[row + 1] - crp[row];
-    sz = wide_split(n);
+    sz = wide_split(n, vals);
     ns = nA > 0 ? (nA + sz - 1) / sz : 1;
     cls = wide_class(n);
 }
@@ -1222,7 +1222,7 @@ DEVI void wide_row_items(const int *rows, int idx, const int *Arp, const long lo
 // pass 1: items per class (per-workgroup counts in LDS, one global atomic per class)
 __global__ __launch_bounds__(256) void k_wide_count(const int *__restrict__ rows, const int *__restrict__ count_p,
                                                     const int *__restrict__ Arp, const long long *__restrict__ crp,
-                                                    unsigned *ctr)
+                                                    int vals, unsigned *ctr)
 {
     __shared__ unsigned lc[WS_CLASSES];
     if (threadIdx.x < WS_CLASSES)
@@ -1231,7 +1231,7 @@ __global__ __launch_bounds__(256) void k_wide_count(const int *__restrict__ rows
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx < *count_p) {
         int row, nA, sz, ns, cls;
-        wide_row_items(rows, idx, Arp, crp, row, nA, sz, ns, cls);
+        wide_row_items(rows, idx, Arp, crp, vals, row, nA, sz, ns, cls);
         atomicAdd(&lc[cls], (unsigned)ns);
     }
     __syncthreads();
@@ -1255,8 +1255,9 @@ __global__ void k_wide_order(unsigned *ctr)
 // pass 3: the items, at per-workgroup ranges of their class
 __global__ __launch_bounds__(256) void k_wide_fill(const int *__restrict__ rows, const int *__restrict__ count_p,
                                                    const int *__restrict__ Arp, const long long *__restrict__ crp,
-                                                   WideItem *__restrict__ items, long long cap, unsigned *ctr,
-                                                   unsigned long long *sctr, long long scratch_n, int *err)
+                                                   int vals, WideItem *__restrict__ items, long long cap,
+                                                   unsigned *ctr, unsigned long long *sctr, long long scratch_n,
+                                                   int *err)
 {
     __shared__ unsigned lc[WS_CLASSES], lb[WS_CLASSES];
     if (threadIdx.x < WS_CLASSES)
@@ -1267,7 +1268,7 @@ __global__ __launch_bounds__(256) void k_wide_fill(const int *__restrict__ rows,
     int row = 0, nA = 0, sz = 1, ns = 0, cls = 0;
     unsigned local = 0;
     if (have) {
-        wide_row_items(rows, idx, Arp, crp, row, nA, sz, ns, cls);
+        wide_row_items(rows, idx, Arp, crp, vals, row, nA, sz, ns, cls);
         local = atomicAdd(&lc[cls], (unsigned)ns);
     }
     __syncthreads();
@@ -1314,7 +1315,8 @@ constexpr int WC_STAGE = 4096; // columns of a flat chunk staged in LDS for coal
 __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ rows, const int *__restrict__ count_p,
                                                      Operands op, const long long *__restrict__ crp,
                                                      int *__restrict__ Ccol, double *__restrict__ Cval,
-                                                     long long sym_row_max, int sym_all_global, int long_rows)
+                                                     long long sym_row_max, int sym_all_global, int long_rows,
+                                                     int vals)
 {
     // one buffer, used by one path per chunk: the staged chunk's columns, or the list of
     // its dense tiles (mask, first slot, absolute tile), so that the waves writing them
@@ -1336,7 +1338,7 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
         const long long n = uniform64(crp[row + 1]) - cb;
         if (n <= 0)
             continue;
-        if (Cval && nA > wide_split(n))
+        if (Cval && nA > wide_split(n, vals)) // a split row: its items add into C
             for (long long q = l; q < n; q += WC_BS)
                 Cval[cb + q] = 0.0;
         const int mn = uniform(op.rmin[row]);
@@ -1425,8 +1427,8 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
 // Wide rows (FP64, directory kept): the compact windowed accumulator of k_numeric_wcompact,
 // on work items pulled from a queue.  An item holds at most WS_SPLIT entries of one row, so
 // their cursors into the B rows live in LDS.  Windows are runs of directory entries that
-// share both k / WS_TILES and (slot of k) / (WS_VALS - 64), so a window holds at most
-// WS_TILES entries and WS_VALS slots; their starts are listed in parallel (WS_LV entries
+// share both k / (VW / 2) and (slot of k) / (VW - 64), so a window holds at most VW / 2
+// entries and VW slots (VW: WS_VALS, or WS_VALS_BIG for sparse rows); their starts are listed in parallel (WS_LV entries
 // per lane, a workgroup scan), WS_WIN at a time.  In a window, the elements of the active
 // B rows are spread over the whole workgroup: each active entry gets chunks of WS_CH
 // elements from its cursor, as many as its density (elements left / columns left to its
@@ -1464,13 +1466,20 @@ constexpr int WS_LV = 8;     // directory entries per lane when listing window s
 constexpr int WS_GL = 16;    // lanes of a chunk group
 constexpr int WS_CH = 4 * WS_GL; // elements of a chunk (WS_GL lanes x 4)
 constexpr int WS_TCAP = 1024; // chunks listed per round
-constexpr int WS_LDS = (WS_VALS * 8 + WS_TILES * 8 + WS_SPLIT * 8 /* av */ + WS_SPLIT * 8 /* pos:col */ +
-                        WS_COARSE * 8 + WS_SPLIT * 4 * 5 /* cur kend last, two act lists */ + WS_COARSE * 4 +
-                        (WS_WIN + 1) * 4 + WS_TCAP * 4 + WS_TILES * 2 + WS_SPLIT * 2 + 15) & ~15;
+// dynamic LDS of a workgroup with windows of vw slots (vw / 2 directory entries)
+constexpr int ws_lds(int vw)
+{
+    return (vw * 8 + (vw / 2) * 8 + WS_SPLIT * 8 /* av */ + WS_SPLIT * 8 /* pos:col */ + WS_COARSE * 8 +
+            WS_SPLIT * 4 * 5 /* cur kend last, two act lists */ + WS_COARSE * 4 + (WS_WIN + 1) * 4 + WS_TCAP * 4 +
+            (vw / 2) * 2 + WS_SPLIT * 2 + 15) & ~15;
+}
+// (the kernel's static LDS is under 256 bytes)
+static_assert(2 * (ws_lds(WS_VALS) + 256) <= 160 * 1024, "two workgroups of the default window per CU");
+static_assert(ws_lds(WS_VALS_BIG) + 256 <= 160 * 1024, "one workgroup of the large window per CU");
 constexpr u64 WS_NOPOS = 0xffffffffull << 32; // (position, column) with no position
 static_assert(2 + 2 * WS_CLASSES <= WIDE_CTRS, "wide-row counters");
 
-template <typename P>
+template <typename P, int VW>
 __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void k_numeric_wsplit(
     const WideItem *__restrict__ items, const unsigned *__restrict__ ctr_listed, unsigned *ctr_taken, Operands op,
     const long long *__restrict__ crp, typename P::V *__restrict__ Cval, u64 *__restrict__ scratch, int *ws,
@@ -1478,12 +1487,13 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
 {
     typedef typename P::V V;
     constexpr bool F32C = std::is_same<P, PrecF32C>::value;
+    constexpr int WT = VW / 2; // directory entries of a window
     const V *__restrict__ Aval = (const V *)op.Aval;
     const V *__restrict__ Bval = (const V *)op.Bval;
     extern __shared__ __align__(16) unsigned char lds[];
     double *vals = (double *)lds;
-    u64 *wm = (u64 *)(vals + WS_VALS);
-    double *av = (double *)(wm + WS_TILES);
+    u64 *wm = (u64 *)(vals + VW);
+    double *av = (double *)(wm + WT);
     u64 *pc = (u64 *)(av + WS_SPLIT); // (position of the next element << 32) | its column
     u64 *cm = pc + WS_SPLIT;
     int *cur = (int *)(cm + WS_COARSE);
@@ -1494,7 +1504,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
     unsigned *wst = co + WS_COARSE;
     unsigned *task = wst + WS_WIN + 1;
     unsigned short *wo = (unsigned short *)(task + WS_TCAP);
-    unsigned short *nch = wo + WS_TILES; // chunks listed for an entry in this round
+    unsigned short *nch = wo + WT; // chunks listed for an entry in this round
     __shared__ int item_s, nact_s[2], ntask_s[2];
     __shared__ unsigned red[16];
     const int l = threadIdx.x, lane = l & 63;
@@ -1578,11 +1588,10 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                         const int k = kl + i;
                         bool st = false;
                         if (k < F) {
-                            if (k == kb || (k % WS_TILES) == 0)
+                            if (k == kb || (k % WT) == 0)
                                 st = true;
                             else
-                                st = (f[i + 1] - off0) / (unsigned)(WS_VALS - 64) !=
-                                     (f[i] - off0) / (unsigned)(WS_VALS - 64);
+                                st = (f[i + 1] - off0) / (unsigned)(VW - 64) != (f[i] - off0) / (unsigned)(VW - 64);
                         }
                         bits |= (st ? 1u : 0u) << i;
                     }
@@ -1939,16 +1948,26 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
             if (e != hipSuccess)
                 return e;
             const unsigned gr = (unsigned)((count + 255) / 256);
-            hipLaunchKernelGGL(k_wide_count, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, wide.ctr);
+            // the large window only for FP64 (the compensated scratch is sized by wide_split
+            // at WS_VALS in the binning)
+            const int vals = !F32C && wide.vals == WS_VALS_BIG ? WS_VALS_BIG : WS_VALS;
+            hipLaunchKernelGGL(k_wide_count, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, vals, wide.ctr);
             hipLaunchKernelGGL(k_wide_order, dim3(1), dim3(64), 0, s, wide.ctr);
-            hipLaunchKernelGGL(k_wide_fill, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, wide.items,
+            hipLaunchKernelGGL(k_wide_fill, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, vals, wide.items,
                                wide.items_cap, wide.ctr, F32C ? wide.sctr : nullptr, wide.scratch_n, err);
             hipLaunchKernelGGL(k_wide_cols, dim3((unsigned)(count < 8192 ? count : 8192)), dim3(WC_BS), 0, s, rows,
                                count_dev, op, crp, Ccol, F32C ? nullptr : (double *)Cval, wide.sym_row_max,
-                               wide.sym_all_global, wide.long_rows);
-            hipLaunchKernelGGL(k_numeric_wsplit<P>, dim3((unsigned)wide.blocks), dim3(WS_BS), WS_LDS, s, wide.items,
-                               wide.ctr, wide.ctr + 1, op, crp, (typename P::V *)Cval, (u64 *)wide.scratch, wide.ws,
-                               wide.ft_stride, wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
+                               wide.sym_all_global, wide.long_rows, vals);
+            if (vals == WS_VALS_BIG)
+                hipLaunchKernelGGL((k_numeric_wsplit<PrecF64, WS_VALS_BIG>), dim3((unsigned)wide.blocks), dim3(WS_BS),
+                                   ws_lds(WS_VALS_BIG), s, wide.items, wide.ctr, wide.ctr + 1, op, crp, (double *)Cval,
+                                   (u64 *)wide.scratch, wide.ws, wide.ft_stride, wide.sym_row_max,
+                                   wide.sym_all_global, wide.long_rows, err);
+            else
+                hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS>), dim3((unsigned)wide.blocks), dim3(WS_BS),
+                                   ws_lds(WS_VALS), s, wide.items, wide.ctr, wide.ctr + 1, op, crp,
+                                   (typename P::V *)Cval, (u64 *)wide.scratch, wide.ws, wide.ft_stride,
+                                   wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
             if (F32C)
                 hipLaunchKernelGGL(k_wide_final, dim3(2048), dim3(256), 0, s, wide.items, wide.ctr, crp, (float *)Cval,
                                    (const u64 *)wide.scratch);
@@ -2019,7 +2038,8 @@ hipError_t configure_kernels(int lds_max, int num_xcds)
 #undef CFGL
     CFG(k_symbolic_wide)
     CFG(k_numeric_window<PrecF64>) CFG(k_numeric_window<PrecF32C>)
-    CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>) CFG(k_numeric_wsplit<PrecF64>) CFG(k_numeric_wsplit<PrecF32C>)
+    CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>) CFG((k_numeric_wsplit<PrecF64, WS_VALS>))
+    CFG((k_numeric_wsplit<PrecF32C, WS_VALS>)) CFG((k_numeric_wsplit<PrecF64, WS_VALS_BIG>))
 #undef CFG
     return e;
 }

@@ -921,7 +921,16 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             if (scratch_max > ar.remaining() / 2) // allocation-free: what the workspace has left
                 scratch_max = ar.remaining() / 2;
             const bool split_items = keep_dirs && (!fp32 || scratch_bytes <= scratch_max);
-            wide.blocks = split_items || cnt > 2 * c->num_cus ? 2 * c->num_cus : cnt;
+            // FP64 items: large windows for sparse rows (few columns per tile spanned)
+            wide.vals = WS_VALS;
+            if (split_items && !fp32) {
+                if (opts.wide_window)
+                    wide.vals = opts.wide_window;
+                else if ((unsigned long long)h2.num.nnz_c[b] < (unsigned long long)WS_BIG_DENSITY * h2.wide_span)
+                    wide.vals = WS_VALS_BIG;
+            }
+            const int per_cu = wide.vals == WS_VALS ? 2 : 1; // workgroups of the window kernel a CU holds
+            wide.blocks = split_items ? per_cu * c->num_cus : (cnt > 2 * c->num_cus ? 2 * c->num_cus : cnt);
             wide.cur_stride = split_items ? 0 : ((long long)h2.num.max_na[b] + 63) & ~63LL;
             wide.ft_stride = keep_dirs ? (((long long)h2.num.max_span[b] + 63) & ~63LL) : 0;
             wide.sym_row_max = bp_sym_row_max;
@@ -1046,6 +1055,8 @@ int check_opts(const hsmu_opts &o)
 {
     if (o.precision != HSMU_FP64 && o.precision != HSMU_FP32_COMP)
         return fail(HSMU_ERR_INVALID, "unknown precision %d", o.precision);
+    if (o.wide_window != 0 && o.wide_window != WS_VALS && o.wide_window != WS_VALS_BIG)
+        return fail(HSMU_ERR_INVALID, "wide_window %d: 0, %d or %d", o.wide_window, WS_VALS, WS_VALS_BIG);
     return HSMU_OK;
 }
 

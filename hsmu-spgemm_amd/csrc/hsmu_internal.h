@@ -67,6 +67,7 @@ struct DevStats {
     int max_brow; // longest row of B (counted from LONG_TILES on)
     unsigned long long num_wide_hint; // rows whose numeric need may exceed LDS (symbolic binning)
     unsigned long long wide_split_nnz; // nnz(C_i) summed over the wide rows split into several items
+    unsigned long long wide_span; // tiles spanned, summed over the wide rows (numeric binning)
     BinStats sym;
     BinStats num;
     // work-queue counters of the persistent LDS kernels: [phase][bin][XCD]
@@ -287,15 +288,20 @@ struct WideItem {
     long long sbase; // compensated FP32, split rows: the row's first (s, c) pair in the scratch
 };
 constexpr int WIDE_CTRS = 66;
-constexpr int WS_VALS = 4096;        // value slots of one window
-constexpr int WS_TILES = 2048;       // directory entries of one window
+// A window holds at most `vals` value slots and vals / 2 directory entries: WS_VALS (two
+// workgroups per CU), or WS_VALS_BIG (one per CU) for FP64 launches whose wide rows hold
+// fewer than WS_BIG_DENSITY columns of C per tile they span (windows there are bounded by
+// directory entries, so larger ones halve the fixed cost per window).
+constexpr int WS_VALS = 4096;
+constexpr int WS_VALS_BIG = 8192;
+constexpr int WS_BIG_DENSITY = 16;
 constexpr int WS_SPLIT = 512;        // A entries of one item, at most
 constexpr int WS_MIN_SPLIT = 64;     // A entries of one item, at least (unless the row has fewer)
 constexpr long long WS_VISITS = 65536; // target of entries x windows per item
-// A entries per item for a row with nA entries and n = nnz(C_i)
-HSMU_HD inline int wide_split(long long n)
+// A entries per item for a row with nA entries and n = nnz(C_i), windows of `vals` slots
+HSMU_HD inline int wide_split(long long n, int vals = WS_VALS)
 {
-    const long long win = n / (WS_VALS - 64) + 1; // windows, about
+    const long long win = n / (vals - 64) + 1; // windows, about
     long long sz = WS_VISITS / win;
     sz = sz < WS_MIN_SPLIT ? WS_MIN_SPLIT : (sz > WS_SPLIT ? WS_SPLIT : sz);
     return (int)(sz & ~63LL);
@@ -308,6 +314,7 @@ struct WideRows {
     long long sym_row_max;
     int sym_all_global;
     int long_rows; // the with_long the symbolic binning used
+    int vals;      // split work items: value slots of a window (WS_VALS or WS_VALS_BIG)
     // split work items (FP64 with kept directories): capacity, list, WIDE_CTRS counters
     // (ctr[0] = items listed, ctr[1] = items taken, then per size class)
     long long items_cap;

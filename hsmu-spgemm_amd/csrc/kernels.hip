@@ -641,6 +641,9 @@ __global__ __launch_bounds__(BS) void k_bin(int numeric, int64_t M, const int *_
             wave_sum<unsigned long long>(b == BIN_GLOBAL && nA > wide_split(nc) ? (unsigned long long)nc : 0ull);
         if (lane == 0 && sn)
             atomicAdd(&st->wide_split_nnz, sn);
+        const unsigned long long ss = wave_sum<unsigned long long>(b == BIN_GLOBAL ? (unsigned long long)S : 0ull);
+        if (lane == 0 && ss)
+            atomicAdd(&st->wide_span, ss);
     }
     int pos = 0;
     long long dloc = -1;

@@ -69,7 +69,7 @@ class Opts(C.Structure):
                 ("products_per_lane", C.c_int32), ("tiles_per_lane", C.c_int32), ("timing", C.c_int32),
                 ("b_lanes", C.c_int32), ("t_lanes", C.c_int32), ("rebuild_dirs", C.c_int32),
                 ("persistent", C.c_int32), ("min_waves", C.c_int32), ("wide_scratch_mb", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("wide_window", C.c_int32)]
 
 
 class Stats(C.Structure):

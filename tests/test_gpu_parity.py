@@ -202,7 +202,7 @@ def split_rows_pair(seed, heavy_rows, na_lo, na_hi, K, N, b_lo, b_hi):
 @pytest.mark.parametrize("shape", [(4, 2000, 3000, 3000, 300000, 20, 80), (1, 20000, 20000, 40000, 1 << 20, 5, 15),
                                    (6, 1500, 2500, 5000, 60000, 100, 400), (3, 3000, 3000, 6000, 1 << 21, 2, 2),
                                    (3, 2500, 2500, 6000, 1 << 23, 2, 2)])
-@pytest.mark.parametrize("opts", [dict(), dict(force_global=1)])
+@pytest.mark.parametrize("opts", [dict(), dict(force_global=1), dict(wide_window=4096), dict(wide_window=8192)])
 def test_split_wide_rows(ctx, shape, opts):
     A, B = split_rows_pair(5, *shape)
     check(ctx, A, B, f"split wide rows {shape} {opts}", **opts)
@@ -362,6 +362,10 @@ def test_errors_fail_loudly(ctx):
     with pytest.raises(hsmu.HsmuError) as e:
         ctx.spgemm(Ab, B)
     assert e.value.status == hsmu.HSMU_ERR_INDEX
+    note("error: wide_window not 0, 4096 or 8192")
+    with pytest.raises(hsmu.HsmuError) as e:
+        ctx.spgemm(A, B, wide_window=6000)
+    assert e.value.status == hsmu.HSMU_ERR_INVALID
     note("after errors the context still works")
     check(ctx, A, B, "after errors")
 
@@ -516,12 +520,13 @@ def test_fp32_compensated_duplicate_columns(ctx):
 
 @pytest.mark.parametrize("shape", [(4, 2000, 3000, 3000, 300000, 20, 80), (6, 1500, 2500, 5000, 60000, 100, 400),
                                    (3, 3000, 3000, 6000, 1 << 21, 2, 2), "rmat12"])
-@pytest.mark.parametrize("opts", [dict(), dict(force_global=1), dict(wide_scratch_mb=1)])
+@pytest.mark.parametrize("opts", [dict(), dict(force_global=1), dict(wide_scratch_mb=1), dict(wide_window=8192)])
 def test_fp32_compensated_split_wide_rows(ctx, shape, opts):
     """Compensated FP32 on wide rows split into several work items: each item's window
     sums meet as (s, c) pairs in a scratch buffer (TwoSum by compare-and-swap) and are
     rounded into C at the end.  With the scratch capped at 1 MiB, rows whose scratch would
-    not fit take one workgroup per row (k_numeric_wcompact) instead."""
+    not fit take one workgroup per row (k_numeric_wcompact) instead.  The large window is
+    an FP64 option: the compensated form keeps 4096 slots whatever wide_window says."""
     if shape == "rmat12":
         A = hsmu.gen_rmat(12, 16, seed=2)
         B = A
