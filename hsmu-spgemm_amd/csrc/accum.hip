@@ -1425,7 +1425,7 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
 }
 
 // Wide rows (FP64, directory kept): the compact windowed accumulator of k_numeric_wcompact,
-// on work items pulled from a queue.  An item holds at most WS_SPLIT entries of one row, so
+// on work items pulled from a queue.  An item holds at most SPLIT entries of one row, so
 // their cursors into the B rows live in LDS.  Windows are runs of directory entries that
 // share both k / (VW / 2) and (slot of k) / (VW - 64), so a window holds at most VW / 2
 // entries and VW slots (VW: WS_VALS, or WS_VALS_BIG for sparse rows); their starts are listed in parallel (WS_LV entries
@@ -1466,12 +1466,14 @@ constexpr int WS_LV = 8;     // directory entries per lane when listing window s
 constexpr int WS_GL = 16;    // lanes of a chunk group
 constexpr int WS_CH = 4 * WS_GL; // elements of a chunk (WS_GL lanes x 4)
 constexpr int WS_TCAP = 1024; // chunks listed per round
-// dynamic LDS of a workgroup with windows of vw slots (vw / 2 directory entries)
+// dynamic LDS of a workgroup with windows of vw slots (vw / 2 directory entries) and items
+// of up to wide_split_max(vw) entries
 constexpr int ws_lds(int vw)
 {
-    return (vw * 8 + (vw / 2) * 8 + WS_SPLIT * 8 /* av */ + WS_SPLIT * 8 /* pos:col */ + WS_COARSE * 8 +
-            WS_SPLIT * 4 * 5 /* cur kend last, two act lists */ + WS_COARSE * 4 + (WS_WIN + 1) * 4 + WS_TCAP * 4 +
-            (vw / 2) * 2 + WS_SPLIT * 2 + 15) & ~15;
+    const int sp = wide_split_max(vw);
+    return (vw * 8 + (vw / 2) * 8 + sp * 8 /* av */ + sp * 8 /* pos:col */ + WS_COARSE * 8 +
+            sp * 4 * 5 /* cur kend last, two act lists */ + WS_COARSE * 4 + (WS_WIN + 1) * 4 + WS_TCAP * 4 +
+            (vw / 2) * 2 + sp * 2 + 15) & ~15;
 }
 // (the kernel's static LDS is under 256 bytes)
 static_assert(2 * (ws_lds(WS_VALS) + 256) <= 160 * 1024, "two workgroups of the default window per CU");
@@ -1488,19 +1490,22 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
     typedef typename P::V V;
     constexpr bool F32C = std::is_same<P, PrecF32C>::value;
     constexpr int WT = VW / 2; // directory entries of a window
+    constexpr int SPLIT = wide_split_max(VW); // entries of an item, at most
+    constexpr int EPL = (SPLIT + WS_BS - 1) / WS_BS; // entries a lane posts and moves
+    static_assert(SPLIT <= 1024 && WS_TCAP <= 1024, "task: entry in bits 0-9, chunk in 10-19");
     const V *__restrict__ Aval = (const V *)op.Aval;
     const V *__restrict__ Bval = (const V *)op.Bval;
     extern __shared__ __align__(16) unsigned char lds[];
     double *vals = (double *)lds;
     u64 *wm = (u64 *)(vals + VW);
     double *av = (double *)(wm + WT);
-    u64 *pc = (u64 *)(av + WS_SPLIT); // (position of the next element << 32) | its column
-    u64 *cm = pc + WS_SPLIT;
+    u64 *pc = (u64 *)(av + SPLIT); // (position of the next element << 32) | its column
+    u64 *cm = pc + SPLIT;
     int *cur = (int *)(cm + WS_COARSE);
-    int *kend = cur + WS_SPLIT;
-    int *last = kend + WS_SPLIT; // last column of B_j
-    int *act = last + WS_SPLIT; // two lists (rounds alternate)
-    unsigned *co = (unsigned *)(act + 2 * WS_SPLIT);
+    int *kend = cur + SPLIT;
+    int *last = kend + SPLIT; // last column of B_j
+    int *act = last + SPLIT; // two lists (rounds alternate)
+    unsigned *co = (unsigned *)(act + 2 * SPLIT);
     unsigned *wst = co + WS_COARSE;
     unsigned *task = wst + WS_WIN + 1;
     unsigned short *wo = (unsigned short *)(task + WS_TCAP);
@@ -1678,8 +1683,8 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                     const int got = off >= WS_TCAP ? 0 : (off + want > WS_TCAP ? WS_TCAP - off : want);
                     nch[e] = (unsigned short)got;
                     for (int c = 0; c < got; ++c)
-                        task[off + c] = (unsigned)e | ((unsigned)c << 9);
-                    act[buf * WS_SPLIT + ab + mbcnt(mh)] = e;
+                        task[off + c] = (unsigned)e | ((unsigned)c << 10);
+                    act[buf * SPLIT + ab + mbcnt(mh)] = e;
                 }
             };
             auto add_c = [&](int c, V v, double a) {
@@ -1694,10 +1699,18 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 P::add(vals, (unsigned)wo[r] + (unsigned)__popcll(wm[r] & below(c & 63)), a, v);
             };
             // round 0: the entries with elements in this window; then those that go again
-            bool has = l < ne && (int)(unsigned)pc[l] < chi;
-            int e = l;
+            // (EPL entries per lane)
+            bool has[EPL];
+            int e[EPL];
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) {
+                e[q] = l + q * WS_BS;
+                has[q] = e[q] < ne && (int)(unsigned)pc[e[q]] < chi;
+            }
             for (int round = 0;; ++round) {
-                post(has, e, round);
+#pragma unroll
+                for (int q = 0; q < EPL; ++q)
+                    post(has[q], e[q], round);
                 __syncthreads();
                 const int buf = round & 1;
                 const int nact = uniform(nact_s[buf]);
@@ -1711,7 +1724,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 }
                 for (int t = l / WS_GL; t < ntask; t += WS_BS / WS_GL) {
                     const unsigned tk = task[t];
-                    const int te = (int)(tk & 511u), tc = (int)(tk >> 9);
+                    const int te = (int)(tk & 1023u), tc = (int)(tk >> 10);
                     const int k = cur[te] + tc * WS_CH;
                     const int ke = kend[te];
                     const double a = av[te];
@@ -1745,26 +1758,31 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 }
                 __syncthreads();
                 // move the cursors; entries whose chunks were all inside the window go again
-                bool again = false;
-                e = -1;
-                if (l < nact) {
-                    e = act[buf * WS_SPLIT + l];
-                    const u64 r = pc[e];
-                    if ((r >> 32) != 0xffffffffull) {
-                        cur[e] = (int)(r >> 32);
-                        pc[e] = WS_NOPOS | (r & 0xffffffffull);
-                    } else {
-                        const int kn = cur[e] + (int)nch[e] * WS_CH;
-                        if (kn >= kend[e]) {
-                            cur[e] = kend[e];
-                            pc[e] = WS_NOPOS | (unsigned)INT_MAX;
+#pragma unroll
+                for (int q = 0; q < EPL; ++q) {
+                    const int idx = l + q * WS_BS;
+                    bool again = false;
+                    int ee = -1;
+                    if (idx < nact) {
+                        ee = act[buf * SPLIT + idx];
+                        const u64 r = pc[ee];
+                        if ((r >> 32) != 0xffffffffull) {
+                            cur[ee] = (int)(r >> 32);
+                            pc[ee] = WS_NOPOS | (r & 0xffffffffull);
                         } else {
-                            cur[e] = kn;
-                            again = true;
+                            const int kn = cur[ee] + (int)nch[ee] * WS_CH;
+                            if (kn >= kend[ee]) {
+                                cur[ee] = kend[ee];
+                                pc[ee] = WS_NOPOS | (unsigned)INT_MAX;
+                            } else {
+                                cur[ee] = kn;
+                                again = true;
+                            }
                         }
                     }
+                    has[q] = again;
+                    e[q] = ee;
                 }
-                has = again;
             }
             // Output: the window's values (its columns were written by k_wide_cols).  Split
             // rows add theirs into C (FP64) or merge their (s, c) pairs into the scratch.

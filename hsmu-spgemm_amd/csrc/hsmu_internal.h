@@ -295,15 +295,21 @@ constexpr int WIDE_CTRS = 66;
 constexpr int WS_VALS = 4096;
 constexpr int WS_VALS_BIG = 8192;
 constexpr int WS_BIG_DENSITY = 16;
-constexpr int WS_SPLIT = 512;        // A entries of one item, at most
+// A entries of one item, at most: WS_SPLIT with the default window, twice as many with
+// the large one (its workgroup has the CU's LDS to itself), so that the fixed cost of a
+// window (its directory, its output) is shared by more entries
+constexpr int WS_SPLIT = 512;
+constexpr int WS_SPLIT_BIG = 1024;
 constexpr int WS_MIN_SPLIT = 64;     // A entries of one item, at least (unless the row has fewer)
-constexpr long long WS_VISITS = 65536; // target of entries x windows per item
+constexpr long long WS_VISITS = 65536; // target of entries x windows per item (per WS_SPLIT)
+HSMU_HD constexpr int wide_split_max(int vals) { return vals == WS_VALS_BIG ? WS_SPLIT_BIG : WS_SPLIT; }
 // A entries per item for a row with nA entries and n = nnz(C_i), windows of `vals` slots
 HSMU_HD inline int wide_split(long long n, int vals = WS_VALS)
 {
     const long long win = n / (vals - 64) + 1; // windows, about
-    long long sz = WS_VISITS / win;
-    sz = sz < WS_MIN_SPLIT ? WS_MIN_SPLIT : (sz > WS_SPLIT ? WS_SPLIT : sz);
+    const long long mx = wide_split_max(vals);
+    long long sz = WS_VISITS * (mx / WS_SPLIT) / win;
+    sz = sz < WS_MIN_SPLIT ? WS_MIN_SPLIT : (sz > mx ? mx : sz);
     return (int)(sz & ~63LL);
 }
 
