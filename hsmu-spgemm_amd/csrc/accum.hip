@@ -1966,9 +1966,9 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
             if (e != hipSuccess)
                 return e;
             const unsigned gr = (unsigned)((count + 255) / 256);
-            // the large window only for FP64 (the compensated scratch is sized by wide_split
-            // at WS_VALS in the binning)
-            const int vals = !F32C && wide.vals == WS_VALS_BIG ? WS_VALS_BIG : WS_VALS;
+            // (the compensated scratch is sized by wide_split at WS_VALS in the binning; the
+            // large window splits a subset of those rows, in fewer items)
+            const int vals = wide.vals == WS_VALS_BIG ? WS_VALS_BIG : WS_VALS;
             hipLaunchKernelGGL(k_wide_count, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, vals, wide.ctr);
             hipLaunchKernelGGL(k_wide_order, dim3(1), dim3(64), 0, s, wide.ctr);
             hipLaunchKernelGGL(k_wide_fill, dim3(gr), dim3(256), 0, s, rows, count_dev, op.Arp, crp, vals, wide.items,
@@ -1977,10 +1977,10 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
                                count_dev, op, crp, Ccol, F32C ? nullptr : (double *)Cval, wide.sym_row_max,
                                wide.sym_all_global, wide.long_rows, vals);
             if (vals == WS_VALS_BIG)
-                hipLaunchKernelGGL((k_numeric_wsplit<PrecF64, WS_VALS_BIG>), dim3((unsigned)wide.blocks), dim3(WS_BS),
-                                   ws_lds(WS_VALS_BIG), s, wide.items, wide.ctr, wide.ctr + 1, op, crp, (double *)Cval,
-                                   (u64 *)wide.scratch, wide.ws, wide.ft_stride, wide.sym_row_max,
-                                   wide.sym_all_global, wide.long_rows, err);
+                hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS_BIG>), dim3((unsigned)wide.blocks), dim3(WS_BS),
+                                   ws_lds(WS_VALS_BIG), s, wide.items, wide.ctr, wide.ctr + 1, op, crp,
+                                   (typename P::V *)Cval, (u64 *)wide.scratch, wide.ws, wide.ft_stride,
+                                   wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
             else
                 hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS>), dim3((unsigned)wide.blocks), dim3(WS_BS),
                                    ws_lds(WS_VALS), s, wide.items, wide.ctr, wide.ctr + 1, op, crp,
@@ -2058,6 +2058,7 @@ hipError_t configure_kernels(int lds_max, int num_xcds)
     CFG(k_numeric_window<PrecF64>) CFG(k_numeric_window<PrecF32C>)
     CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>) CFG((k_numeric_wsplit<PrecF64, WS_VALS>))
     CFG((k_numeric_wsplit<PrecF32C, WS_VALS>)) CFG((k_numeric_wsplit<PrecF64, WS_VALS_BIG>))
+    CFG((k_numeric_wsplit<PrecF32C, WS_VALS_BIG>))
 #undef CFG
     return e;
 }

@@ -921,9 +921,9 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             if (scratch_max > ar.remaining() / 2) // allocation-free: what the workspace has left
                 scratch_max = ar.remaining() / 2;
             const bool split_items = keep_dirs && (!fp32 || scratch_bytes <= scratch_max);
-            // FP64 items: large windows for sparse rows (few columns per tile spanned)
+            // split items: large windows for sparse rows (few columns per tile spanned)
             wide.vals = WS_VALS;
-            if (split_items && !fp32) {
+            if (split_items) {
                 if (opts.wide_window)
                     wide.vals = opts.wide_window;
                 else if ((unsigned long long)h2.num.nnz_c[b] < (unsigned long long)WS_BIG_DENSITY * h2.wide_span)

@@ -289,7 +289,7 @@ struct WideItem {
 };
 constexpr int WIDE_CTRS = 66;
 // A window holds at most `vals` value slots and vals / 2 directory entries: WS_VALS (two
-// workgroups per CU), or WS_VALS_BIG (one per CU) for FP64 launches whose wide rows hold
+// workgroups per CU), or WS_VALS_BIG (one per CU) for launches whose wide rows hold
 // fewer than WS_BIG_DENSITY columns of C per tile they span (windows there are bounded by
 // directory entries, so larger ones halve the fixed cost per window).
 constexpr int WS_VALS = 4096;

@@ -525,8 +525,9 @@ def test_fp32_compensated_split_wide_rows(ctx, shape, opts):
     """Compensated FP32 on wide rows split into several work items: each item's window
     sums meet as (s, c) pairs in a scratch buffer (TwoSum by compare-and-swap) and are
     rounded into C at the end.  With the scratch capped at 1 MiB, rows whose scratch would
-    not fit take one workgroup per row (k_numeric_wcompact) instead.  The large window is
-    an FP64 option: the compensated form keeps 4096 slots whatever wide_window says."""
+    not fit take one workgroup per row (k_numeric_wcompact) instead.  wide_window=8192
+    forces the large window (items of up to 1,024 entries), whose split rows are a subset
+    of those the scratch was sized for."""
     if shape == "rmat12":
         A = hsmu.gen_rmat(12, 16, seed=2)
         B = A
