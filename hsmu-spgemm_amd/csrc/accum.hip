@@ -256,6 +256,8 @@ struct PrecF32C {
     static DEVI void add(double *vals, unsigned slot, double a, V b) { put(vals, slot, prep(a, b)); }
     static DEVI void put(double *vals, unsigned slot, T pe)
     {
+        // no contraction: t must be fl(s + p), not fma(a, b, s), for the error to be exact
+#pragma clang fp contract(off)
         const float p = pe.p, e = pe.e;
         unsigned long long *w = (unsigned long long *)vals + slot;
         unsigned long long old = *w;
