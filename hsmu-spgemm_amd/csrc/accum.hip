@@ -1483,8 +1483,17 @@ static_assert(ws_lds(WS_VALS_BIG) + 256 <= 160 * 1024, "one workgroup of the lar
 constexpr u64 WS_NOPOS = 0xffffffffull << 32; // (position, column) with no position
 static_assert(2 + 2 * WS_CLASSES <= WIDE_CTRS, "wide-row counters");
 
+// Lanes of a workgroup: WS_BS with the default window (two workgroups per CU, 6 waves
+// per SIMD); 512 with the large one, whose workgroup has the CU to itself (2 waves per
+// SIMD).  Fewer waves there are faster: the window's phases share one barrier and one
+// LDS, whose atomics queue up (R-MAT scale 20 wide-row bin: 256 lanes 323 ms, 384 lanes
+// 288 ms, 512 lanes 289 ms, 768 lanes 330 ms, 1,024 lanes 384 ms; compensated FP32
+// 386 / 323 / 309 / 333 / 372 ms).
+template <int VW> constexpr int ws_bs() { return VW == WS_VALS_BIG ? 512 : WS_BS; }
+template <int VW> constexpr int ws_waves() { return VW == WS_VALS_BIG ? 2 : 6; }
+
 template <typename P, int VW>
-__global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void k_numeric_wsplit(
+__global__ __launch_bounds__(ws_bs<VW>()) __attribute__((amdgpu_waves_per_eu(ws_waves<VW>()))) void k_numeric_wsplit(
     const WideItem *__restrict__ items, const unsigned *__restrict__ ctr_listed, unsigned *ctr_taken, Operands op,
     const long long *__restrict__ crp, typename P::V *__restrict__ Cval, u64 *__restrict__ scratch, int *ws,
     long long ft_stride, long long sym_row_max, int sym_all_global, int long_rows, int *err)
@@ -1492,8 +1501,9 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
     typedef typename P::V V;
     constexpr bool F32C = std::is_same<P, PrecF32C>::value;
     constexpr int WT = VW / 2; // directory entries of a window
+    constexpr int BS = ws_bs<VW>();
     constexpr int SPLIT = wide_split_max(VW); // entries of an item, at most
-    constexpr int EPL = (SPLIT + WS_BS - 1) / WS_BS; // entries a lane posts and moves
+    constexpr int EPL = (SPLIT + BS - 1) / BS; // entries a lane posts and moves
     static_assert(SPLIT <= 1024 && WS_TCAP <= 1024, "task: entry in bits 0-9, chunk in 10-19");
     const V *__restrict__ Aval = (const V *)op.Aval;
     const V *__restrict__ Bval = (const V *)op.Bval;
@@ -1543,14 +1553,14 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
         const u64 *fm = op.dir_mask + rd + W;
         const unsigned *foff = op.dir_off + rd + W;
         const bool clds = W <= WS_COARSE;
-        for (int q = l; q < W && clds; q += WS_BS) {
+        for (int q = l; q < W && clds; q += BS) {
             cm[q] = cmg[q];
             co[q] = cog[q];
         }
         const int F = two ? (int)(uniform(cog[W - 1]) + (unsigned)__popcll(uniform64(cmg[W - 1]))) : S;
         const unsigned off0 = F > 0 ? uniform(foff[0]) : 0u;
         if (two) {
-            for (int q = l; q < W; q += WS_BS) {
+            for (int q = l; q < W; q += BS) {
                 u64 m = cmg[q];
                 unsigned k = cog[q];
                 while (m) {
@@ -1559,7 +1569,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 }
             }
         }
-        for (int e = l; e < ne; e += WS_BS) {
+        for (int e = l; e < ne; e += BS) {
             const int j = op.Acol[a0 + ie0 + e];
             const int k0 = op.Brp[j], k1 = op.Brp[j + 1];
             cur[e] = k0;
@@ -1580,8 +1590,8 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                     break;
                 int cnt_st = 0; // starts listed so far
                 // WS_LV consecutive entries per lane, their offsets loaded together: one
-                // round trip covers WS_BS * WS_LV entries
-                for (int kk = kb; kk < F && cnt_st <= WS_WIN; kk += WS_BS * WS_LV) {
+                // round trip covers BS * WS_LV entries
+                for (int kk = kb; kk < F && cnt_st <= WS_WIN; kk += BS * WS_LV) {
                     const int kl = kk + l * WS_LV;
                     unsigned f[WS_LV + 1]; // foff[kl - 1 + i]
 #pragma unroll
@@ -1603,7 +1613,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                         bits |= (st ? 1u : 0u) << i;
                     }
                     unsigned tot;
-                    unsigned pos = group_excl_scan<WS_BS>((unsigned)__popc(bits), red, tot);
+                    unsigned pos = group_excl_scan<BS>((unsigned)__popc(bits), red, tot);
                     while (bits) { // this lane's starts, in order
                         if (cnt_st + (int)pos <= WS_WIN)
                             wst[cnt_st + pos] = (unsigned)(kl + __ffs(bits) - 1);
@@ -1639,11 +1649,11 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 nact_s[0] = 0;
                 ntask_s[0] = 0;
             }
-            for (int q = l; q < nk; q += WS_BS) {
+            for (int q = l; q < nk; q += BS) {
                 wm[q] = fm[k0 + q];
                 wo[q] = (unsigned short)(foff[k0 + q] - base);
             }
-            for (int q = l; q < cnt; q += WS_BS)
+            for (int q = l; q < cnt; q += BS)
                 vals[q] = 0.0;
             const int lastt = two ? uniform(ft[k1 - 1]) : k1 - 1;
             const int chi = (mn + lastt + 1) << 6; // end of the window's last tile
@@ -1706,7 +1716,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
             int e[EPL];
 #pragma unroll
             for (int q = 0; q < EPL; ++q) {
-                e[q] = l + q * WS_BS;
+                e[q] = l + q * BS;
                 has[q] = e[q] < ne && (int)(unsigned)pc[e[q]] < chi;
             }
             for (int round = 0;; ++round) {
@@ -1724,7 +1734,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                     nact_s[buf ^ 1] = 0;
                     ntask_s[buf ^ 1] = 0;
                 }
-                for (int t = l / WS_GL; t < ntask; t += WS_BS / WS_GL) {
+                for (int t = l / WS_GL; t < ntask; t += BS / WS_GL) {
                     const unsigned tk = task[t];
                     const int te = (int)(tk & 1023u), tc = (int)(tk >> 10);
                     const int k = cur[te] + tc * WS_CH;
@@ -1762,7 +1772,7 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
                 // move the cursors; entries whose chunks were all inside the window go again
 #pragma unroll
                 for (int q = 0; q < EPL; ++q) {
-                    const int idx = l + q * WS_BS;
+                    const int idx = l + q * BS;
                     bool again = false;
                     int ee = -1;
                     if (idx < nact) {
@@ -1791,23 +1801,23 @@ __global__ __launch_bounds__(WS_BS) __attribute__((amdgpu_waves_per_eu(6))) void
             if constexpr (F32C) {
                 const float *fv = (const float *)vals;
                 if (split) {
-                    for (int q = l; q < cnt; q += WS_BS) {
+                    for (int q = l; q < cnt; q += BS) {
                         const float sv = fv[2 * q], cv = fv[2 * q + 1];
                         if (sv != 0.0f || cv != 0.0f)
                             merge_pair(&scratch[sbase + base + q], sv, cv);
                     }
                 } else {
-                    for (int q = l; q < cnt; q += WS_BS)
+                    for (int q = l; q < cnt; q += BS)
                         Cval[cb + base + q] = fv[2 * q] + fv[2 * q + 1];
                 }
             } else if (split) {
-                for (int q = l; q < cnt; q += WS_BS) {
+                for (int q = l; q < cnt; q += BS) {
                     const double v = vals[q];
                     if (v != 0.0)
                         unsafeAtomicAdd(&Cval[cb + base + q], v);
                 }
             } else {
-                for (int q = l; q < cnt; q += WS_BS)
+                for (int q = l; q < cnt; q += BS)
                     Cval[cb + base + q] = vals[q];
             }
             done += cnt;
@@ -1979,7 +1989,7 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
                                count_dev, op, crp, Ccol, F32C ? nullptr : (double *)Cval, wide.sym_row_max,
                                wide.sym_all_global, wide.long_rows, vals);
             if (vals == WS_VALS_BIG)
-                hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS_BIG>), dim3((unsigned)wide.blocks), dim3(WS_BS),
+                hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS_BIG>), dim3((unsigned)wide.blocks), dim3(ws_bs<WS_VALS_BIG>()),
                                    ws_lds(WS_VALS_BIG), s, wide.items, wide.ctr, wide.ctr + 1, op, crp,
                                    (typename P::V *)Cval, (u64 *)wide.scratch, wide.ws, wide.ft_stride,
                                    wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
