@@ -1483,17 +1483,21 @@ static_assert(ws_lds(WS_VALS_BIG) + 256 <= 160 * 1024, "one workgroup of the lar
 constexpr u64 WS_NOPOS = 0xffffffffull << 32; // (position, column) with no position
 static_assert(2 + 2 * WS_CLASSES <= WIDE_CTRS, "wide-row counters");
 
-// Lanes of a workgroup: WS_BS with the default window (two workgroups per CU, 6 waves
-// per SIMD); 512 with the large one, whose workgroup has the CU to itself (2 waves per
-// SIMD).  Fewer waves there are faster: the window's phases share one barrier and one
-// LDS, whose atomics queue up (R-MAT scale 20 wide-row bin: 256 lanes 323 ms, 384 lanes
-// 288 ms, 512 lanes 289 ms, 768 lanes 330 ms, 1,024 lanes 384 ms; compensated FP32
-// 386 / 323 / 309 / 333 / 372 ms).
-template <int VW> constexpr int ws_bs() { return VW == WS_VALS_BIG ? 512 : WS_BS; }
-template <int VW> constexpr int ws_waves() { return VW == WS_VALS_BIG ? 2 : 6; }
+// Lanes of a workgroup.  With the large window the workgroup has the CU to itself and
+// takes 512 lanes (2 waves per SIMD); fewer waves are faster there, since a window's
+// phases share one barrier and one LDS whose atomics queue up (R-MAT scale 20 wide-row
+// bin: 256 lanes 323 ms, 384 lanes 288 ms, 512 lanes 289 ms, 768 lanes 330 ms, 1,024
+// lanes 384 ms; compensated FP32 386 / 323 / 309 / 333 / 372 ms).  With the default
+// window two workgroups share a CU: 512 lanes for FP64 (a Zipf panel's wide rows 39.4 ->
+// 37.2 ms against 768), WS_BS for the compensated form (34.6 ms against 35.0 at 512).
+template <typename P, int VW> constexpr int ws_bs()
+{
+    return VW == WS_VALS_BIG || std::is_same<P, PrecF64>::value ? 512 : WS_BS;
+}
+template <typename P, int VW> constexpr int ws_waves() { return VW == WS_VALS_BIG ? 2 : (ws_bs<P, VW>() == 512 ? 4 : 6); }
 
 template <typename P, int VW>
-__global__ __launch_bounds__(ws_bs<VW>()) __attribute__((amdgpu_waves_per_eu(ws_waves<VW>()))) void k_numeric_wsplit(
+__global__ __launch_bounds__((ws_bs<P, VW>())) __attribute__((amdgpu_waves_per_eu((ws_waves<P, VW>())))) void k_numeric_wsplit(
     const WideItem *__restrict__ items, const unsigned *__restrict__ ctr_listed, unsigned *ctr_taken, Operands op,
     const long long *__restrict__ crp, typename P::V *__restrict__ Cval, u64 *__restrict__ scratch, int *ws,
     long long ft_stride, long long sym_row_max, int sym_all_global, int long_rows, int *err)
@@ -1501,7 +1505,7 @@ __global__ __launch_bounds__(ws_bs<VW>()) __attribute__((amdgpu_waves_per_eu(ws_
     typedef typename P::V V;
     constexpr bool F32C = std::is_same<P, PrecF32C>::value;
     constexpr int WT = VW / 2; // directory entries of a window
-    constexpr int BS = ws_bs<VW>();
+    constexpr int BS = ws_bs<P, VW>();
     constexpr int SPLIT = wide_split_max(VW); // entries of an item, at most
     constexpr int EPL = (SPLIT + BS - 1) / BS; // entries a lane posts and moves
     static_assert(SPLIT <= 1024 && WS_TCAP <= 1024, "task: entry in bits 0-9, chunk in 10-19");
@@ -1989,12 +1993,12 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
                                count_dev, op, crp, Ccol, F32C ? nullptr : (double *)Cval, wide.sym_row_max,
                                wide.sym_all_global, wide.long_rows, vals);
             if (vals == WS_VALS_BIG)
-                hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS_BIG>), dim3((unsigned)wide.blocks), dim3(ws_bs<WS_VALS_BIG>()),
+                hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS_BIG>), dim3((unsigned)wide.blocks), dim3(ws_bs<P, WS_VALS_BIG>()),
                                    ws_lds(WS_VALS_BIG), s, wide.items, wide.ctr, wide.ctr + 1, op, crp,
                                    (typename P::V *)Cval, (u64 *)wide.scratch, wide.ws, wide.ft_stride,
                                    wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
             else
-                hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS>), dim3((unsigned)wide.blocks), dim3(WS_BS),
+                hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS>), dim3((unsigned)wide.blocks), dim3(ws_bs<P, WS_VALS>()),
                                    ws_lds(WS_VALS), s, wide.items, wide.ctr, wide.ctr + 1, op, crp,
                                    (typename P::V *)Cval, (u64 *)wide.scratch, wide.ws, wide.ft_stride,
                                    wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
