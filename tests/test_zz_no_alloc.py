@@ -156,3 +156,55 @@ def test_noalloc_limits(fresh):
         for x in (tight, fits, obig, dA, dbig):
             x.free()
         ctx.trim()
+
+
+def wide_pair(seed, heavy_rows, na_lo, na_hi, K, N, b_lo, b_hi):
+    """A (heavy_rows + 100 rows x K) whose first rows hold na_lo..na_hi entries, B (K x N)
+    with b_lo..b_hi entries per row: the first rows of C are wider than LDS and are split
+    into several work items of the windowed accumulator."""
+    rng = np.random.default_rng(seed)
+    rows_b = [np.sort(rng.choice(N, size=int(rng.integers(b_lo, b_hi + 1)), replace=False)).astype(np.int32)
+              for _ in range(K)]
+    rp_b = np.zeros(K + 1, np.int32)
+    rp_b[1:] = np.cumsum([len(r) for r in rows_b])
+    B = hsmu.HostMatrix(K, N, rp_b, np.concatenate(rows_b), rng.uniform(-1, 1, int(rp_b[-1])))
+    M = heavy_rows + 100
+    rows_a = [np.sort(rng.choice(K, size=int(rng.integers(na_lo, na_hi + 1)) if i < heavy_rows
+                                 else int(rng.integers(0, 31)), replace=False)).astype(np.int32) for i in range(M)]
+    rp_a = np.zeros(M + 1, np.int32)
+    rp_a[1:] = np.cumsum([len(r) for r in rows_a])
+    A = hsmu.HostMatrix(M, K, rp_a, np.concatenate(rows_a), rng.uniform(-1, 1, int(rp_a[-1])))
+    return A, B
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_noalloc_wide_rows_both_windows(fresh, fp32):
+    """Wide rows split into work items, through the allocation-free call: sparse ones
+    (about two columns per tile: the 8,192-slot window) and then denser ones (the
+    4,096-slot window), each against hsmu_spgemm and the oracle."""
+    ctx = fresh
+    as32 = lambda m: hsmu.HostMatrix(m.M, m.N, m.rowptr, m.col, m.val.astype(np.float32).astype(np.float64))
+    A1, B1 = wide_pair(11, 3, 3000, 6000, 6000, 1 << 21, 2, 2)
+    A2, B2 = wide_pair(12, 4, 2000, 3000, 3000, 300000, 20, 80)
+    if fp32:
+        A1, B1, A2, B2 = as32(A1), as32(B1), as32(A2), as32(B2)
+    kw = dict(precision=hsmu.HSMU_FP32_COMP) if fp32 else {}
+    d = [ctx.matrix(m, fp32) for m in (A1, B1, A2, B2)]
+    n1 = int(oracle.spgemm(A1, B1).rowptr[-1])
+    n2 = int(oracle.spgemm(A2, B2).rowptr[-1])
+    o1 = Outputs(ctx, A1.M, n1, fp32=fp32)
+    o2 = Outputs(ctx, A2.M, n2, fp32=fp32)
+    try:
+        note(f"noalloc wide rows ({'fp32c' if fp32 else 'fp64'}): sparse nnz(C) {n1}, then denser nnz(C) {n2}")
+        rc, c1, st1 = noalloc(ctx, d[0], d[1], o1, **kw)
+        assert rc == hsmu.HSMU_OK, hsmu.lib.hsmu_last_error()
+        rc, c2, st2 = noalloc(ctx, d[2], d[3], o2, **kw)
+        assert rc == hsmu.HSMU_OK, hsmu.lib.hsmu_last_error()
+        assert hsmu.lib.hsmu_ctx_pool_bytes(ctx.handle) == 0
+        assert st1["num_bin_rows"][8] >= 1 and st2["num_bin_rows"][8] >= 1  # rows took the windowed path
+        check_same(ctx, A1, B1, c1, fp32=fp32, **kw)
+        check_same(ctx, A2, B2, c2, fp32=fp32, **kw)
+    finally:
+        for x in [o1, o2] + d:
+            x.free()
+        ctx.trim()
