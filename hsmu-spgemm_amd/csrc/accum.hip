@@ -1442,25 +1442,6 @@ __global__ __launch_bounds__(WC_BS) void k_wide_cols(const int *__restrict__ row
 // in LDS when there are at most WS_COARSE of them.  Items of a split row add their window
 // sums into C (zeroed by k_wide_cols, which also writes every wide row's columns) with
 // atomics; the other rows store them.
-// (S, C) += (s, c) on a compensated pair in global memory, one 64-bit compare-and-swap
-// per attempt: TwoSum of S and s (as in PrecF32C::add), its error and c go into C.
-DEVI void merge_pair(u64 *w, float s, float c)
-{
-    u64 old = *w;
-    while (true) {
-        const float S = __uint_as_float((unsigned)old), Cc = __uint_as_float((unsigned)(old >> 32));
-        const float t = S + s;
-        const float bb = t - S;
-        const float err = (S - (t - bb)) + (s - bb);
-        const float c2 = Cc + (err + c);
-        const u64 nw = (u64)__float_as_uint(t) | ((u64)__float_as_uint(c2) << 32);
-        const u64 prev = atomicCAS((unsigned long long *)w, (unsigned long long)old, (unsigned long long)nw);
-        if (prev == old)
-            break;
-        old = prev;
-    }
-}
-
 constexpr int WS_COARSE = 256;
 constexpr int WS_BS = 768;   // lanes of a workgroup (two fit a CU: 6 waves per SIMD)
 constexpr int WS_WIN = 256;  // window starts listed at a time
@@ -1801,14 +1782,17 @@ __global__ __launch_bounds__((ws_bs<P, VW>())) __attribute__((amdgpu_waves_per_e
                 }
             }
             // Output: the window's values (its columns were written by k_wide_cols).  Split
-            // rows add theirs into C (FP64) or merge their (s, c) pairs into the scratch.
+            // rows add theirs into C (FP64) or, compensated, s + c into the FP64 scratch.
             if constexpr (F32C) {
                 const float *fv = (const float *)vals;
                 if (split) {
+                    // the item's compensated sums meet in FP64 (s + c is exact in a double),
+                    // one atomic add each, no retries; rounded to FP32 once, at the end
+                    double *sd = (double *)scratch + sbase + base;
                     for (int q = l; q < cnt; q += BS) {
                         const float sv = fv[2 * q], cv = fv[2 * q + 1];
                         if (sv != 0.0f || cv != 0.0f)
-                            merge_pair(&scratch[sbase + base + q], sv, cv);
+                            unsafeAtomicAdd(&sd[q], (double)sv + (double)cv);
                     }
                 } else {
                     for (int q = l; q < cnt; q += BS)
@@ -1833,8 +1817,9 @@ __global__ __launch_bounds__((ws_bs<P, VW>())) __attribute__((amdgpu_waves_per_e
     }
 }
 
-// Compensated FP32, rows split into several items: C's values from the merged (s, c)
-// pairs.  A workgroup per row; the row's first item names it and its scratch range.
+// Compensated FP32, rows split into several items: C's values, rounded once from the
+// FP64 sums the items added into the scratch.  A workgroup per row; the row's first item
+// names it and its scratch range.
 __global__ __launch_bounds__(256) void k_wide_final(const WideItem *__restrict__ items,
                                                     const unsigned *__restrict__ ctr_listed,
                                                     const long long *__restrict__ crp, float *__restrict__ Cval,
@@ -1848,10 +1833,9 @@ __global__ __launch_bounds__(256) void k_wide_final(const WideItem *__restrict__
         const int row = uniform(items[idx].row);
         const long long sb = uniform64(items[idx].sbase);
         const long long cb = uniform64(crp[row]), n = uniform64(crp[row + 1]) - cb;
-        for (long long q = threadIdx.x; q < n; q += 256) {
-            const u64 w = scratch[sb + q];
-            Cval[cb + q] = __uint_as_float((unsigned)w) + __uint_as_float((unsigned)(w >> 32));
-        }
+        const double *sd = (const double *)scratch + sb;
+        for (long long q = threadIdx.x; q < n; q += 256)
+            Cval[cb + q] = (float)sd[q];
     }
 }
 

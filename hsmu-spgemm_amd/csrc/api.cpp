@@ -914,7 +914,7 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
         // WS_MIN_SPLIT of them) on a full persistent grid.
         WideRows wide = {};
         if (b == BIN_GLOBAL) {
-            // the compensated form's split rows need an (s, c) pair per nonzero of theirs
+            // the compensated form's split rows need an FP64 sum per nonzero of theirs
             const unsigned long long scratch_bytes = h2.wide_split_nnz * sizeof(unsigned long long);
             unsigned long long scratch_max =
                 opts.wide_scratch_mb > 0 ? (unsigned long long)opts.wide_scratch_mb << 20 : c->total_mem / 4;

@@ -285,7 +285,7 @@ hipError_t launch_scan_i32_i64(const int *in, int64_t n, long long *out, void *s
 // (nsplit > 1); their partial sums meet in C's values by atomic adds (C zeroed first).
 struct WideItem {
     int row, e0, e1, nsplit;
-    long long sbase; // compensated FP32, split rows: the row's first (s, c) pair in the scratch
+    long long sbase; // compensated FP32, split rows: the row's first sum in the scratch
 };
 constexpr int WIDE_CTRS = 66;
 // A window holds at most `vals` value slots and vals / 2 directory entries: WS_VALS (two
@@ -326,9 +326,9 @@ struct WideRows {
     long long items_cap;
     WideItem *items;
     unsigned *ctr;
-    // compensated FP32 split rows: one (s, c) float pair per nonzero of those rows (zeroed
-    // before the items merge their windows into it), and the counter that hands out each
-    // row's range
+    // compensated FP32 split rows: one FP64 sum per nonzero of those rows (zeroed before
+    // the items add their windows into it), and the counter that hands out each row's
+    // range
     unsigned long long *scratch;
     long long scratch_n;
     unsigned long long *sctr;

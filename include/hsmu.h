@@ -128,7 +128,7 @@ typedef struct hsmu_opts {
     int32_t min_waves;        /* a row that has a workgroup to itself gets enough lanes for
                                  this many waves per CU at the occupancy its LDS allows (at
                                  most 1024 lanes); 0 = 8 */
-    int32_t wide_scratch_mb;  /* compensated FP32: MiB the (s, c) scratch of wide rows split
+    int32_t wide_scratch_mb;  /* compensated FP32: MiB the FP64 scratch of wide rows split
                                  into several work items may take; 0 = a quarter of device
                                  memory.  Beyond it those rows take one workgroup each */
     int32_t wide_window;      /* rows wider than LDS, split into work items: value slots of
