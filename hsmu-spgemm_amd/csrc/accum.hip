@@ -1464,21 +1464,28 @@ static_assert(ws_lds(WS_VALS_BIG) + 256 <= 160 * 1024, "one workgroup of the lar
 constexpr u64 WS_NOPOS = 0xffffffffull << 32; // (position, column) with no position
 static_assert(2 + 2 * WS_CLASSES <= WIDE_CTRS, "wide-row counters");
 
-// Lanes of a workgroup.  With the large window the workgroup has the CU to itself and
-// takes 512 lanes (2 waves per SIMD); fewer waves are faster there, since a window's
-// phases share one barrier and one LDS whose atomics queue up (R-MAT scale 20 wide-row
-// bin: 256 lanes 323 ms, 384 lanes 288 ms, 512 lanes 289 ms, 768 lanes 330 ms, 1,024
-// lanes 384 ms; compensated FP32 386 / 323 / 309 / 333 / 372 ms).  With the default
-// window two workgroups share a CU: 512 lanes for FP64 (a Zipf panel's wide rows 39.4 ->
-// 37.2 ms against 768), WS_BS for the compensated form (34.6 ms against 35.0 at 512).
-template <typename P, int VW> constexpr int ws_bs()
+// Lanes of a workgroup, by form, window and density.  The default window runs two
+// workgroups per CU, the large one a single workgroup.  A window's phases share one
+// barrier and one LDS whose atomics queue up, so fewer waves than the CU could hold are
+// faster, the more so the sparser the rows (R-MAT scale 20, wide-row bin, default window:
+// FP64 128 / 256 / 512 lanes 355 / 263 / 290 ms, compensated FP32 256 / 384 / 768 lanes
+// 286 / 276 / 339 ms; large window, 512 lanes: 287 and 301 ms).  Dense rows want more
+// lanes (a Zipf A·Aᵀ panel's wide rows, FP64 256 / 384 / 512 / 768 lanes 41.4 / 38.0 /
+// 37.2 / 39.4 ms; compensated 384 / 512 / 768 lanes 38.2 / 35.0 / 34.6 ms).
+template <typename P, int VW, bool SPARSE> constexpr int ws_bs()
 {
-    return VW == WS_VALS_BIG || std::is_same<P, PrecF64>::value ? 512 : WS_BS;
+    return VW == WS_VALS_BIG ? 512
+                             : (std::is_same<P, PrecF64>::value ? (SPARSE ? 256 : 512) : (SPARSE ? 384 : WS_BS));
 }
-template <typename P, int VW> constexpr int ws_waves() { return VW == WS_VALS_BIG ? 2 : (ws_bs<P, VW>() == 512 ? 4 : 6); }
+// waves per SIMD the launch holds (its workgroups per CU x waves / 4 SIMDs), for the
+// compiler's register budget
+template <int VW, int BS> constexpr int ws_waves()
+{
+    return (BS / 64) * (VW == WS_VALS_BIG ? 1 : 2) / 4 > 0 ? (BS / 64) * (VW == WS_VALS_BIG ? 1 : 2) / 4 : 1;
+}
 
-template <typename P, int VW>
-__global__ __launch_bounds__((ws_bs<P, VW>())) __attribute__((amdgpu_waves_per_eu((ws_waves<P, VW>())))) void k_numeric_wsplit(
+template <typename P, int VW, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu((ws_waves<VW, BS>())))) void k_numeric_wsplit(
     const WideItem *__restrict__ items, const unsigned *__restrict__ ctr_listed, unsigned *ctr_taken, Operands op,
     const long long *__restrict__ crp, typename P::V *__restrict__ Cval, u64 *__restrict__ scratch, int *ws,
     long long ft_stride, long long sym_row_max, int sym_all_global, int long_rows, int *err)
@@ -1486,7 +1493,6 @@ __global__ __launch_bounds__((ws_bs<P, VW>())) __attribute__((amdgpu_waves_per_e
     typedef typename P::V V;
     constexpr bool F32C = std::is_same<P, PrecF32C>::value;
     constexpr int WT = VW / 2; // directory entries of a window
-    constexpr int BS = ws_bs<P, VW>();
     constexpr int SPLIT = wide_split_max(VW); // entries of an item, at most
     constexpr int EPL = (SPLIT + BS - 1) / BS; // entries a lane posts and moves
     static_assert(SPLIT <= 1024 && WS_TCAP <= 1024, "task: entry in bits 0-9, chunk in 10-19");
@@ -1976,16 +1982,18 @@ static hipError_t numeric_p(int bin, const int *rows, const int *count_dev, int 
             hipLaunchKernelGGL(k_wide_cols, dim3((unsigned)(count < 8192 ? count : 8192)), dim3(WC_BS), 0, s, rows,
                                count_dev, op, crp, Ccol, F32C ? nullptr : (double *)Cval, wide.sym_row_max,
                                wide.sym_all_global, wide.long_rows, vals);
+#define WSPLIT(VW, SPARSE)                                                                                           \
+    hipLaunchKernelGGL((k_numeric_wsplit<P, VW, ws_bs<P, VW, SPARSE>()>), dim3((unsigned)wide.blocks),                 \
+                       dim3(ws_bs<P, VW, SPARSE>()), ws_lds(VW), s, wide.items, wide.ctr, wide.ctr + 1, op, crp,        \
+                       (typename P::V *)Cval, (u64 *)wide.scratch, wide.ws, wide.ft_stride, wide.sym_row_max,           \
+                       wide.sym_all_global, wide.long_rows, err)
             if (vals == WS_VALS_BIG)
-                hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS_BIG>), dim3((unsigned)wide.blocks), dim3(ws_bs<P, WS_VALS_BIG>()),
-                                   ws_lds(WS_VALS_BIG), s, wide.items, wide.ctr, wide.ctr + 1, op, crp,
-                                   (typename P::V *)Cval, (u64 *)wide.scratch, wide.ws, wide.ft_stride,
-                                   wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
+                WSPLIT(WS_VALS_BIG, false);
+            else if (wide.sparse)
+                WSPLIT(WS_VALS, true);
             else
-                hipLaunchKernelGGL((k_numeric_wsplit<P, WS_VALS>), dim3((unsigned)wide.blocks), dim3(ws_bs<P, WS_VALS>()),
-                                   ws_lds(WS_VALS), s, wide.items, wide.ctr, wide.ctr + 1, op, crp,
-                                   (typename P::V *)Cval, (u64 *)wide.scratch, wide.ws, wide.ft_stride,
-                                   wide.sym_row_max, wide.sym_all_global, wide.long_rows, err);
+                WSPLIT(WS_VALS, false);
+#undef WSPLIT
             if (F32C)
                 hipLaunchKernelGGL(k_wide_final, dim3(2048), dim3(256), 0, s, wide.items, wide.ctr, crp, (float *)Cval,
                                    (const u64 *)wide.scratch);
@@ -2056,9 +2064,13 @@ hipError_t configure_kernels(int lds_max, int num_xcds)
 #undef CFGL
     CFG(k_symbolic_wide)
     CFG(k_numeric_window<PrecF64>) CFG(k_numeric_window<PrecF32C>)
-    CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>) CFG((k_numeric_wsplit<PrecF64, WS_VALS>))
-    CFG((k_numeric_wsplit<PrecF32C, WS_VALS>)) CFG((k_numeric_wsplit<PrecF64, WS_VALS_BIG>))
-    CFG((k_numeric_wsplit<PrecF32C, WS_VALS_BIG>))
+    CFG(k_numeric_wcompact<PrecF64>) CFG(k_numeric_wcompact<PrecF32C>) 
+#define CFGW(P)                                                                                                        \
+    CFG((k_numeric_wsplit<P, WS_VALS, ws_bs<P, WS_VALS, false>()>))                                                    \
+    CFG((k_numeric_wsplit<P, WS_VALS, ws_bs<P, WS_VALS, true>()>))                                                     \
+    CFG((k_numeric_wsplit<P, WS_VALS_BIG, ws_bs<P, WS_VALS_BIG, false>()>))
+    CFGW(PrecF64) CFGW(PrecF32C)
+#undef CFGW
 #undef CFG
     return e;
 }

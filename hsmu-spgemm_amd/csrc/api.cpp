@@ -921,14 +921,11 @@ int pipeline(hsmu_ctx *c, const Problem &P, const hsmu_opts &opts, Arena &ar, co
             if (scratch_max > ar.remaining() / 2) // allocation-free: what the workspace has left
                 scratch_max = ar.remaining() / 2;
             const bool split_items = keep_dirs && (!fp32 || scratch_bytes <= scratch_max);
-            // split items: large windows for sparse rows (few columns per tile spanned)
-            wide.vals = WS_VALS;
-            if (split_items) {
-                if (opts.wide_window)
-                    wide.vals = opts.wide_window;
-                else if ((unsigned long long)h2.num.nnz_c[b] < (unsigned long long)WS_BIG_DENSITY * h2.wide_span)
-                    wide.vals = WS_VALS_BIG;
-            }
+            // split items: the window size asked for (the default one otherwise), and
+            // narrow workgroups for sparse rows (few columns per tile spanned)
+            wide.vals = split_items && opts.wide_window ? opts.wide_window : WS_VALS;
+            wide.sparse =
+                (unsigned long long)h2.num.nnz_c[b] < (unsigned long long)WS_SPARSE_DENSITY * h2.wide_span ? 1 : 0;
             const int per_cu = wide.vals == WS_VALS ? 2 : 1; // workgroups of the window kernel a CU holds
             wide.blocks = split_items ? per_cu * c->num_cus : (cnt > 2 * c->num_cus ? 2 * c->num_cus : cnt);
             wide.cur_stride = split_items ? 0 : ((long long)h2.num.max_na[b] + 63) & ~63LL;

@@ -289,12 +289,12 @@ struct WideItem {
 };
 constexpr int WIDE_CTRS = 66;
 // A window holds at most `vals` value slots and vals / 2 directory entries: WS_VALS (two
-// workgroups per CU), or WS_VALS_BIG (one per CU) for launches whose wide rows hold
-// fewer than WS_BIG_DENSITY columns of C per tile they span (windows there are bounded by
-// directory entries, so larger ones halve the fixed cost per window).
+// workgroups per CU, the default), or WS_VALS_BIG (one per CU, on request).  Launches
+// whose wide rows hold fewer than WS_SPARSE_DENSITY columns of C per tile they span take
+// narrower workgroups (the sparse lane counts of k_numeric_wsplit).
 constexpr int WS_VALS = 4096;
 constexpr int WS_VALS_BIG = 8192;
-constexpr int WS_BIG_DENSITY = 16;
+constexpr int WS_SPARSE_DENSITY = 16;
 // A entries of one item, at most: WS_SPLIT with the default window, twice as many with
 // the large one (its workgroup has the CU's LDS to itself), so that the fixed cost of a
 // window (its directory, its output) is shared by more entries
@@ -321,6 +321,7 @@ struct WideRows {
     int sym_all_global;
     int long_rows; // the with_long the symbolic binning used
     int vals;      // split work items: value slots of a window (WS_VALS or WS_VALS_BIG)
+    int sparse;    // split work items: the wide rows hold few columns per tile (narrow workgroups)
     // split work items (FP64 with kept directories): capacity, list, WIDE_CTRS counters
     // (ctr[0] = items listed, ctr[1] = items taken, then per size class)
     long long items_cap;

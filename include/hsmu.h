@@ -132,9 +132,10 @@ typedef struct hsmu_opts {
                                  into several work items may take; 0 = a quarter of device
                                  memory.  Beyond it those rows take one workgroup each */
     int32_t wide_window;      /* rows wider than LDS, split into work items: value slots of
-                                 one window of the windowed accumulator, 4096 or 8192 (one
-                                 workgroup per CU); 0 = 8192 when those rows hold fewer than
-                                 16 columns of C per 64-column tile they span, else 4096 */
+                                 one window of the windowed accumulator, 4096 (two
+                                 workgroups per CU) or 8192 (one); 0 = 4096.  Workgroups
+                                 are narrower when those rows hold fewer than 16 columns of
+                                 C per 64-column tile they span */
 } hsmu_opts;
 
 #define HSMU_NBINS 10 /* 8 LDS bins (8..1024 lanes per row), global fallback, wide rows */

@@ -180,8 +180,9 @@ def wide_pair(seed, heavy_rows, na_lo, na_hi, K, N, b_lo, b_hi):
 @pytest.mark.parametrize("fp32", [False, True])
 def test_noalloc_wide_rows_both_windows(fresh, fp32):
     """Wide rows split into work items, through the allocation-free call: sparse ones
-    (about two columns per tile: the 8,192-slot window) and then denser ones (the
-    4,096-slot window), each against hsmu_spgemm and the oracle."""
+    (about two columns per tile: narrow workgroups), the same through the 8,192-slot
+    window (wide_window=8192), then denser ones (wide workgroups), each against
+    hsmu_spgemm and the oracle."""
     ctx = fresh
     as32 = lambda m: hsmu.HostMatrix(m.M, m.N, m.rowptr, m.col, m.val.astype(np.float32).astype(np.float64))
     A1, B1 = wide_pair(11, 3, 3000, 6000, 6000, 1 << 21, 2, 2)
@@ -193,18 +194,24 @@ def test_noalloc_wide_rows_both_windows(fresh, fp32):
     n1 = int(oracle.spgemm(A1, B1).rowptr[-1])
     n2 = int(oracle.spgemm(A2, B2).rowptr[-1])
     o1 = Outputs(ctx, A1.M, n1, fp32=fp32)
+    o1b = Outputs(ctx, A1.M, n1, fp32=fp32)
     o2 = Outputs(ctx, A2.M, n2, fp32=fp32)
     try:
-        note(f"noalloc wide rows ({'fp32c' if fp32 else 'fp64'}): sparse nnz(C) {n1}, then denser nnz(C) {n2}")
+        note(f"noalloc wide rows ({'fp32c' if fp32 else 'fp64'}): sparse nnz(C) {n1} (default, then 8,192-slot "
+             f"window), then denser nnz(C) {n2}")
         rc, c1, st1 = noalloc(ctx, d[0], d[1], o1, **kw)
+        assert rc == hsmu.HSMU_OK, hsmu.lib.hsmu_last_error()
+        rc, c1b, st1b = noalloc(ctx, d[0], d[1], o1b, wide_window=8192, **kw)
         assert rc == hsmu.HSMU_OK, hsmu.lib.hsmu_last_error()
         rc, c2, st2 = noalloc(ctx, d[2], d[3], o2, **kw)
         assert rc == hsmu.HSMU_OK, hsmu.lib.hsmu_last_error()
         assert hsmu.lib.hsmu_ctx_pool_bytes(ctx.handle) == 0
-        assert st1["num_bin_rows"][8] >= 1 and st2["num_bin_rows"][8] >= 1  # rows took the windowed path
+        for st in (st1, st1b, st2):
+            assert st["num_bin_rows"][8] >= 1  # rows took the windowed path
         check_same(ctx, A1, B1, c1, fp32=fp32, **kw)
+        check_same(ctx, A1, B1, c1b, fp32=fp32, **kw)
         check_same(ctx, A2, B2, c2, fp32=fp32, **kw)
     finally:
-        for x in [o1, o2] + d:
+        for x in [o1, o1b, o2] + d:
             x.free()
         ctx.trim()
