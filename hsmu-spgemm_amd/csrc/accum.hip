@@ -1493,6 +1493,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu((ws_waves<VW
     typedef typename P::V V;
     constexpr bool F32C = std::is_same<P, PrecF32C>::value;
     constexpr int WT = VW / 2; // directory entries of a window
+    // chunk tasks a 16-lane group loads before it adds any: a narrow (sparse-row)
+    // workgroup has few groups, each with many tasks per round, and the registers for
+    // four (R-MAT scale 20 wide-row bin 263 -> 244 ms FP64, 275 -> 265 ms compensated;
+    // eight: no better for FP64, 384 ms compensated; two for the dense FP64 launch: within
+    // 1% on a Zipf panel)
+    constexpr int NTS = BS < 512 ? 4 : 1;
     constexpr int SPLIT = wide_split_max(VW); // entries of an item, at most
     constexpr int EPL = (SPLIT + BS - 1) / BS; // entries a lane posts and moves
     static_assert(SPLIT <= 1024 && WS_TCAP <= 1024, "task: entry in bits 0-9, chunk in 10-19");
@@ -1725,38 +1731,57 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu((ws_waves<VW
                     nact_s[buf ^ 1] = 0;
                     ntask_s[buf ^ 1] = 0;
                 }
-                for (int t = l / WS_GL; t < ntask; t += BS / WS_GL) {
-                    const unsigned tk = task[t];
-                    const int te = (int)(tk & 1023u), tc = (int)(tk >> 10);
-                    const int k = cur[te] + tc * WS_CH;
-                    const int ke = kend[te];
-                    const double a = av[te];
-                    int c[4];
-                    V v[4];
+                // a group loads the chunks of NTS tasks before it adds any of them
+                constexpr int NG = BS / WS_GL;
+                for (int t = l / WS_GL; t < ntask; t += NTS * NG) {
+                    int te[NTS], k[NTS], ke[NTS];
+                    double a[NTS];
+                    int c[NTS][4];
+                    V v[NTS][4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int p = k + ll + WS_GL * u;
-                        c[u] = INT_MAX;
-                        v[u] = 0;
-                        if (p < ke) {
-                            c[u] = op.Bcol[p];
-                            v[u] = Bval[p];
+                    for (int h = 0; h < NTS; ++h) {
+                        const int th = t + h * NG;
+                        te[h] = 0;
+                        k[h] = 0;
+                        ke[h] = 0; // no task: no element loaded
+                        a[h] = 0.0;
+                        if (th < ntask) {
+                            const unsigned tk = task[th];
+                            te[h] = (int)(tk & 1023u);
+                            k[h] = cur[te[h]] + (int)(tk >> 10) * WS_CH;
+                            ke[h] = kend[te[h]];
+                            a[h] = av[te[h]];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int p = k[h] + ll + WS_GL * u;
+                            c[h][u] = INT_MAX;
+                            v[h][u] = 0;
+                            if (p < ke[h]) {
+                                c[h][u] = op.Bcol[p];
+                                v[h][u] = Bval[p];
+                            }
                         }
                     }
-                    int took = 0;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const bool in = c[u] < chi;
-                        if (in)
-                            add_c(c[u], v[u], a);
-                        took += __popcll((__ballot(in) >> lg0) & ((1ull << WS_GL) - 1ull));
-                    }
-                    if (took < WS_CH) { // the prefix ends here: offer (position, column) after it
+                    for (int h = 0; h < NTS; ++h) {
+                        if (t + h * NG >= ntask) // uniform in the group (t is)
+                            break;
+                        int took = 0;
 #pragma unroll
-                        for (int u = 0; u < 4; ++u)
-                            if (ll + WS_GL * u == took)
-                                atomicMin((unsigned long long *)&pc[te],
-                                          ((u64)(unsigned)(k + took) << 32) | (unsigned)c[u]);
+                        for (int u = 0; u < 4; ++u) {
+                            const bool in = c[h][u] < chi;
+                            if (in)
+                                add_c(c[h][u], v[h][u], a[h]);
+                            took += __popcll((__ballot(in) >> lg0) & ((1ull << WS_GL) - 1ull));
+                        }
+                        if (took < WS_CH) { // the prefix ends here: offer (position, column) after it
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                if (ll + WS_GL * u == took)
+                                    atomicMin((unsigned long long *)&pc[te[h]],
+                                              ((u64)(unsigned)(k[h] + took) << 32) | (unsigned)c[h][u]);
+                        }
                     }
                 }
                 __syncthreads();
