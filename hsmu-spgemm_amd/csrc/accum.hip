@@ -1467,15 +1467,17 @@ static_assert(2 + 2 * WS_CLASSES <= WIDE_CTRS, "wide-row counters");
 // Lanes of a workgroup, by form, window and density.  The default window runs two
 // workgroups per CU, the large one a single workgroup.  A window's phases share one
 // barrier and one LDS whose atomics queue up, so fewer waves than the CU could hold are
-// faster, the more so the sparser the rows (R-MAT scale 20, wide-row bin, default window:
-// FP64 128 / 256 / 512 lanes 355 / 263 / 290 ms, compensated FP32 256 / 384 / 768 lanes
-// 286 / 276 / 339 ms; large window, 512 lanes: 287 and 301 ms).  Dense rows want more
-// lanes (a Zipf A·Aᵀ panel's wide rows, FP64 256 / 384 / 512 / 768 lanes 41.4 / 38.0 /
-// 37.2 / 39.4 ms; compensated 384 / 512 / 768 lanes 38.2 / 35.0 / 34.6 ms).
+// faster, the more so the sparser the rows (R-MAT scale 20, wide-row bin, default window,
+// four chunk tasks in flight per group: FP64 192 / 256 / 320 lanes 258 / 246 / 340 ms,
+// compensated FP32 256 / 320 / 384 lanes 266 / 261 / 265 ms; one task at a time: FP64
+// 128 / 256 / 512 lanes 355 / 263 / 290 ms; large window, 512 lanes: 287 and 301 ms).
+// Dense rows want more lanes (a Zipf A·Aᵀ panel's wide rows, FP64 256 / 384 / 512 / 768
+// lanes 41.4 / 38.0 / 37.2 / 39.4 ms; compensated 384 / 512 / 768 lanes 38.2 / 35.0 /
+// 34.6 ms).
 template <typename P, int VW, bool SPARSE> constexpr int ws_bs()
 {
     return VW == WS_VALS_BIG ? 512
-                             : (std::is_same<P, PrecF64>::value ? (SPARSE ? 256 : 512) : (SPARSE ? 384 : WS_BS));
+                             : (std::is_same<P, PrecF64>::value ? (SPARSE ? 256 : 512) : (SPARSE ? 320 : WS_BS));
 }
 // waves per SIMD the launch holds (its workgroups per CU x waves / 4 SIMDs), for the
 // compiler's register budget
